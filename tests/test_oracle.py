@@ -1,0 +1,242 @@
+"""Pins of the CPU oracle: against the reference's own outputs (golden
+fixtures written from the reference C) and against what the mathematics fixes
+(mpmath brute force, closed forms, invariants).  CPU only."""
+import numpy as np
+import pytest
+
+G = 4 * np.pi ** 2
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+
+
+def same_bits(a, b):
+    """Bit equality, with any NaN equal to any NaN."""
+    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all((bits(a) == bits(b)) | nan))
+
+
+def run_chunks(orc, pos, vel, m, r, t_end, store_dt, Gc=G, ext=None, skip=False):
+    """abie/integrator.py:158-203 driving integrator_gr per store_dt chunk."""
+    s = orc.Sim(pos, vel, m, r, G=Gc, ext_acc=ext, skip_first_node=skip)
+    t = 0.0
+    dt = min(store_dt, t_end - t)
+    rows = []
+    e0 = s.energy()
+    while t < t_end:
+        next_t = t + dt - ((t + dt) % dt)
+        ret = s.integrate(next_t, t)
+        t = s.t
+        rows.append((next_t, t, ret, s.pos.copy(), s.vel.copy(), s.energy()))
+        if ret != 0:
+            break
+    return s, e0, rows
+
+
+# ------------------------------------------------------------- tables
+def test_tables_bitwise_equal_reference(orc, golden):
+    ref = golden("radau_tables")
+    hs, rs, cs = orc.tables()
+    assert same_bits(hs, ref["hs"]) and same_bits(rs, ref["rs"]) and same_bits(cs, ref["cs"])
+    assert ref["epsb"] == 1e-4 and ref["fac"] == 0.25 and ref["tolpc"] == 1e-16
+    assert ref["exponent"] == 1.0 / 7 and ref["h_min"] == 1e-13
+
+
+def test_tables_from_definition(orc):
+    import mpmath as mp
+    mp.mp.dps = 40
+    hs, rs, cs = orc.tables()
+    for h in hs:
+        v = mp.legendre(7, 2 * mp.mpf(h) - 1) + mp.legendre(8, 2 * mp.mpf(h) - 1)
+        assert abs(v) < 1e-13
+    for j in range(8):
+        for k in range(j):
+            assert rs[j, k] == pytest.approx(1.0 / (hs[j] - hs[k]), rel=1e-15)
+    # row j of cs: coefficients of prod_{i<=j} (x - h_i); evaluate at a point
+    for j in range(1, 8):
+        x = 0.37
+        poly = np.prod([x - hs[i] for i in range(1, j + 1)])
+        assert np.polyval(cs[j, : j + 1][::-1], x) == pytest.approx(poly, rel=1e-13, abs=1e-16)
+
+
+# ------------------------------------------------------------- forces
+@pytest.mark.parametrize("case,Gk", [("c0", "G"), ("c1a", "G"), ("pl", 1.0)])
+def test_forces_bitwise_reference(orc, golden, case, Gk):
+    d = golden("forces")
+    Gc = float(d["G"]) if Gk == "G" else Gk
+    acc = orc.accel(d[case + "_pos"], Gc, d[case + "_m"])
+    assert same_bits(acc, d[case + "_acc"])
+
+
+@pytest.mark.parametrize("case", ["one", "mix", "coinc", "tpon", "wide"])
+def test_forces_edge_reference(orc, golden, case):
+    d = golden("forces_edge")
+    init = d[case + "_init"] if case + "_init" in d else None
+    acc = orc.accel(d[case + "_pos"], float(d["G"]), d[case + "_m"], acc_init=init)
+    assert same_bits(acc, d[case + "_acc"])
+
+
+def test_forces_mpmath_bruteforce(orc):
+    import mpmath as mp
+    mp.mp.dps = 50
+    rng = np.random.default_rng(5)
+    for n in (2, 3, 7, 16):
+        pos = rng.normal(size=(n, 3)) * rng.uniform(0.1, 10)
+        m = np.exp(rng.uniform(np.log(1e-9), 0, n)); m[n // 2] = 0.0
+        acc = orc.accel(pos, G, m)
+        for j in range(n):
+            a = [mp.mpf(0)] * 3
+            for k in range(n):
+                if k == j or m[k] <= 0:
+                    continue
+                d = [mp.mpf(pos[j, c]) - mp.mpf(pos[k, c]) for c in range(3)]
+                r = mp.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+                for c in range(3):
+                    a[c] -= mp.mpf(G) * mp.mpf(m[k]) * d[c] / r ** 3
+            an = float(mp.sqrt(sum(x ** 2 for x in a)))
+            err = np.sqrt(sum((acc[j, c] - float(a[c])) ** 2 for c in range(3)))
+            scale = sum(G * m[k] / np.sum((pos[j] - pos[k]) ** 2) for k in range(n) if k != j and m[k] > 0)
+            assert err <= 1e-15 * max(an, 1e-300) + 4e-16 * scale
+
+
+def test_forces_closed_form_and_invariants(orc):
+    # two bodies: a = -G M r / |r|^3
+    pos = np.array([[0.0, 0.0, 0.0], [3.0, 4.0, 0.0]]); m = np.array([2.0, 0.5])
+    acc = orc.accel(pos, G, m)
+    assert acc[1] == pytest.approx(-G * 2.0 * np.array([3, 4, 0]) / 125.0, rel=1e-15)
+    assert acc[0] == pytest.approx(G * 0.5 * np.array([3, 4, 0]) / 125.0, rel=1e-15)
+    rng = np.random.default_rng(9)
+    pos = rng.normal(size=(40, 3)); m = rng.uniform(0.1, 1.0, 40); m[30:] = 0.0
+    acc = orc.accel(pos, G, m)
+    # momentum: sum m a = 0 over massive bodies
+    assert np.abs((m[:, None] * acc).sum(0)).max() <= 1e-12 * np.abs(m[:, None] * acc).sum()
+    # linearity in G, inverse-square scaling
+    np.testing.assert_allclose(orc.accel(pos, 2 * G, m), 2 * acc, rtol=1e-14)
+    np.testing.assert_allclose(orc.accel(3.0 * pos, G, m), acc / 9.0, rtol=1e-13)
+    # test particles never act on massive bodies
+    np.testing.assert_array_equal(orc.accel(pos[:30], G, m[:30]), acc[:30])
+
+
+def test_div_const_correctly_rounded(orc):
+    """The product's GR15 predictor divides by the constants of
+    integrator_gauss_radau15.c:58,65 with Markstein's residual correction;
+    here that method is checked against IEEE division bit for bit."""
+    L = orc.lib()
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.normal(size=20000) * 10.0 ** rng.uniform(-300, 300, 20000),
+                         np.array([0.0, -0.0, 1.0, -1.0, 3.0, 1e-300, -7.5e300])])
+    for c in (2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0, 10.0, 15.0, 21.0, 28.0, 36.0):
+        rc = 1.0 / c
+        got = np.array([L.orc_div_const(float(x), c, rc) for x in xs])
+        assert same_bits(got, xs / c), c
+
+
+# ------------------------------------------------------------- GR15
+@pytest.mark.parametrize("case,t_end,store_dt", [("small", 1.0, 1.0), ("chunks", 1.0, 0.25),
+                                                  ("c0", 5.0, 2.5), ("ext", 0.5, 0.5)])
+def test_gr15_bitwise_reference(orc, golden, case, t_end, store_dt):
+    d = golden("gr15")
+    pre = "small" if case == "chunks" else case
+    m = d[pre + "_m"]
+    ext = d["ext_acc"] if case == "ext" else None
+    _, e0, rows = run_chunks(orc, d[pre + "_pos0"], d[pre + "_vel0"], m, np.zeros_like(m),
+                             t_end, store_dt, ext=ext)
+    assert e0 == d[case + "_e0"]
+    assert len(rows) == len(d[case + "_t"])
+    for i, (nt, t, ret, pos, vel, en) in enumerate(rows):
+        assert nt == d[case + "_next_t"][i]
+        assert t == d[case + "_t"][i]
+        assert ret == d[case + "_ret"][i]
+        assert same_bits(pos, d[case + "_pos"][i])
+        assert same_bits(vel, d[case + "_vel"][i])
+        assert en == d[case + "_energy"][i]
+
+
+def test_gr15_skip_first_node_is_bit_identical(orc, golden):
+    d = golden("gr15")
+    m = d["c0_m"]
+    s1, _, r1 = run_chunks(orc, d["c0_pos0"], d["c0_vel0"], m, np.zeros_like(m), 5.0, 2.5)
+    s2, _, r2 = run_chunks(orc, d["c0_pos0"], d["c0_vel0"], m, np.zeros_like(m), 5.0, 2.5, skip=True)
+    for a, b in zip(r1, r2):
+        assert a[1] == b[1] and same_bits(a[3], b[3]) and same_bits(a[4], b[4])
+    c1, c2 = s1.counters(), s2.counters()
+    assert c1["attempts"] == c2["attempts"] and c2["evals"] < c1["evals"]
+
+
+def kepler_solution(r0, v0, mu, t):
+    """Universal-variable two-body propagation in mpmath (closed form)."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    r0 = [mp.mpf(x) for x in r0]; v0 = [mp.mpf(x) for x in v0]
+    mu = mp.mpf(mu)
+    rn = mp.sqrt(sum(x * x for x in r0)); vn2 = sum(x * x for x in v0)
+    a = 1 / (2 / rn - vn2 / mu)
+    n = mp.sqrt(mu / a ** 3)
+    rv = sum(x * y for x, y in zip(r0, v0))
+    e_cos = 1 - rn / a; e_sin = rv / mp.sqrt(mu * a)
+    e = mp.sqrt(e_cos ** 2 + e_sin ** 2)
+    E0 = mp.atan2(e_sin, e_cos)
+    M = E0 - e_sin + n * mp.mpf(t)
+    E = M
+    for _ in range(60):
+        E = E - (E - e * mp.sin(E) - M) / (1 - e * mp.cos(E))
+    dE = E - E0
+    F = 1 - a / rn * (1 - mp.cos(dE))
+    Gt = mp.mpf(t) - mp.sqrt(a ** 3 / mu) * (dE - mp.sin(dE))
+    return [float(F * r0[i] + Gt * v0[i]) for i in range(3)]
+
+
+def test_gr15_kepler_closed_form(orc):
+    mu = G * 1.0
+    r0 = np.array([1.0, 0.0, 0.0]); v0 = np.array([0.0, 2 * np.pi * 1.1, 0.0])  # e ~ 0.21
+    pos = np.array([[0.0, 0.0, 0.0], r0]); vel = np.array([[0.0, 0.0, 0.0], v0])
+    m = np.array([1.0, 0.0])
+    s = orc.Sim(pos, vel, m)
+    s.integrate(10.0)
+    got = s.pos.reshape(-1, 3)[1]
+    want = kepler_solution(r0, v0, mu, s.t)
+    assert np.linalg.norm(got - np.array(want)) < 1e-9
+
+
+def test_gr15_energy_conservation_c0_like(orc):
+    from abie_amd import ics
+    pos, vel, m = ics.config0(seed=3, n_test=50)
+    s = orc.Sim(pos, vel, m)
+    e0 = s.energy()
+    s.integrate(3.0)
+    assert abs((s.energy() - e0) / e0) < 1e-13
+
+
+# ------------------------------------------------------------- events, energy
+def test_events_check_reference(orc, golden):
+    d = golden("events_check")
+    s = orc.Sim(d["chk_pos"], np.zeros_like(d["chk_pos"]), np.ones(5), d["chk_r"],
+                max_ce=3, max_coll=2, ce_dist=float(d["chk_ce_dist"]))
+    for i, t in enumerate(d["chk_times"]):
+        assert s.check_events(t=float(t)) == d["chk_codes"][i]
+        assert same_bits(s.buf_ce, d["chk_ce"][i]) and same_bits(s.buf_coll, d["chk_coll"][i])
+
+
+def test_events_gr15_reference(orc, golden):
+    d = golden("events_gr15")
+    s = orc.Sim(d["coll_pos0"], d["coll_vel0"], d["coll_m"], d["coll_r"], max_ce=5, max_coll=1)
+    ret = s.integrate(1.0, 0.0)
+    assert ret == d["coll_ret"][0] == 2
+    assert s.t == d["coll_t"][0]
+    assert same_bits(s.pos, d["coll_pos"][0]) and same_bits(s.vel, d["coll_vel"][0])
+    assert same_bits(s.buf_coll, d["coll_buf"])
+
+
+def test_energy_reference_and_closed_form(orc, golden):
+    d = golden("energy")
+    s = orc.Sim(d["pos"], d["vel"], d["m"])
+    assert s.energy() == d["energy"]
+    # two bodies on a circular orbit about their barycentre
+    m1, m2, a = 1.0, 1e-3, 2.0
+    v = np.sqrt(G * (m1 + m2) / a)
+    pos = np.array([[-a * m2 / (m1 + m2), 0, 0], [a * m1 / (m1 + m2), 0, 0]])
+    vel = np.array([[0, -v * m2 / (m1 + m2), 0], [0, v * m1 / (m1 + m2), 0]])
+    s = orc.Sim(pos, vel, np.array([m1, m2]))
+    assert s.energy() == pytest.approx(-G * m1 * m2 / (2 * a), rel=1e-13)
