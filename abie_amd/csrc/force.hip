@@ -1,0 +1,292 @@
+// force.hip -- all-pairs Newtonian accelerations on gfx950.
+//
+// Replaces src/common.c:39-77 (CPU) and src/gpuforce.cu:60-84 (CUDA).
+// Layout: sources and sinks in SoA double arrays in HBM.  Sources are the
+// massive bodies only (m > 0), compacted in ascending body order, with
+// gm = G*m precomputed; sinks [0, n_self) are those same bodies, sinks beyond
+// are test particles.  A 256-thread block owns BLOCK*SPT sinks (SPT in
+// registers, strided by BLOCK so loads and stores coalesce) and streams the
+// sources through LDS in tiles of BLOCK; every lane of a wave reads the same
+// LDS word (broadcast).  Per-sink sums run in ascending source order.
+//
+// Interaction (ABIE_FORCE_FP64), 16 VALU ops + 1 v_rsq_f64:
+//   d = p_i - s_k (3)   r2 = |d|^2 (1 mul + 2 fma)
+//   y = rsq(r2); e = 1 - r2 y^2; s = gm y^3 (1 + 3/2 e + 15/8 e^2)   (7)
+//   a -= s d (3 fma)
+// The rsq seed error e enters only at third order (35/16 e^3), so the term is
+// accurate to a few ulp as long as the seed is good to ~2^-20.
+#include "devmath.hpp"
+#include "force.hpp"
+
+namespace abie {
+
+// Correction coefficients held in registers: gfx950 VOP3 takes no literal,
+// and a literal forces a v_fmac with two v_mov per interaction.
+struct Coef {
+    double c1, c2;  // 3/2, 15/8
+};
+
+__device__ __forceinline__ Coef make_coef() {
+    Coef k{1.5, 1.875};
+    asm volatile("" : "+v"(k.c1), "+v"(k.c2));
+    return k;
+}
+
+template <int MODE>
+struct Interact;
+
+// FP64: rsqrt seed + second-order correction, FMA accumulation.
+template <>
+struct Interact<ABIE_FORCE_FP64> {
+    static __device__ __forceinline__ void run(double px, double py, double pz, double sx, double sy,
+                                               double sz, double gm, double &ax, double &ay, double &az, Coef k) {
+        const double dx = px - sx, dy = py - sy, dz = pz - sz;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double y = __builtin_amdgcn_rsq(r2);
+        const double y2 = y * y;
+        const double e = fma(-r2, y2, 1.0);
+        const double g3 = (gm * y) * y2;
+        const double p = fma(e, k.c2, k.c1);
+        const double s = fma(g3 * e, p, g3);
+        ax = fma(-s, dx, ax);
+        ay = fma(-s, dy, ay);
+        az = fma(-s, dz, az);
+    }
+};
+
+// Reference operation order, src/common.c:62-70: sqrt, r*r2, (gm*d)/r3.
+template <>
+struct Interact<ABIE_FORCE_FP64_EXACT> {
+    static __device__ __forceinline__ void run(double px, double py, double pz, double sx, double sy,
+                                               double sz, double gm, double &ax, double &ay, double &az, Coef) {
+        const double dx = px - sx, dy = py - sy, dz = pz - sz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r3 = sqrt(r2) * r2;
+        ax = ax - gm * dx / r3;
+        ay = ay - gm * dy / r3;
+        az = az - gm * dz / r3;
+    }
+};
+
+// Test particle from a massive body: FP64 relative coordinates, FP32 r^-3 by
+// v_rsq_f32 and one Newton step, FP64 accumulation of the FP32 term.
+template <>
+struct Interact<ABIE_FORCE_MIXED> {
+    static __device__ __forceinline__ void run(double px, double py, double pz, double sx, double sy,
+                                               double sz, double gm, double &ax, double &ay, double &az, Coef) {
+        const float dx = (float)(px - sx), dy = (float)(py - sy), dz = (float)(pz - sz);
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        float y = __builtin_amdgcn_rsqf(r2);
+        const float hy = 0.5f * y;
+        const float e = fmaf(-r2 * y, y, 1.0f);
+        y = fmaf(hy, e, y);
+        const float s = ((float)gm * y) * (y * y);
+        ax = ax - (double)(s * dx);
+        ay = ay - (double)(s * dy);
+        az = az - (double)(s * dz);
+    }
+};
+
+constexpr int BLOCK = 256;
+
+// Sum over sources [k0, k1) of one LDS tile for SPT sinks.  CHECK: compare
+// the source index with each sink's own index (diagonal tiles only).
+template <int SPT, int MODE, bool CHECK>
+__device__ __forceinline__ void tile_sum(const double *__restrict__ tx, const double *__restrict__ ty,
+                                         const double *__restrict__ tz, const double *__restrict__ tg,
+                                         int cnt, int kbase, const int (&idx)[SPT],
+                                         const double (&px)[SPT], const double (&py)[SPT],
+                                         const double (&pz)[SPT], double (&ax)[SPT], double (&ay)[SPT],
+                                         double (&az)[SPT], Coef kc) {
+    if (!CHECK && cnt == BLOCK) {
+#pragma unroll 4
+        for (int k = 0; k < BLOCK; ++k) {
+            const double sx = tx[k], sy = ty[k], sz = tz[k], g = tg[k];
+#pragma unroll
+            for (int s = 0; s < SPT; ++s) Interact<MODE>::run(px[s], py[s], pz[s], sx, sy, sz, g, ax[s], ay[s], az[s], kc);
+        }
+        return;
+    }
+#pragma unroll 2
+    for (int k = 0; k < cnt; ++k) {
+        const double sx = tx[k], sy = ty[k], sz = tz[k], g = tg[k];
+#pragma unroll
+        for (int s = 0; s < SPT; ++s) {
+            if (!CHECK || idx[s] != kbase + k)
+                Interact<MODE>::run(px[s], py[s], pz[s], sx, sy, sz, g, ax[s], ay[s], az[s], kc);
+        }
+    }
+}
+
+// One block: BLOCK*SPT sinks against sources [j0, j1).  OUT_PARTIAL writes
+// to part[(p*3 + c)*n_sink + i] for the j-split reduction instead of a.
+template <int SPT, int MODE, bool OUT_PARTIAL>
+__global__ __launch_bounds__(BLOCK) void accel_kernel(AccelArgs a, int j_chunk, double *__restrict__ part) {
+    __shared__ double tx[BLOCK], ty[BLOCK], tz[BLOCK], tg[BLOCK];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * BLOCK * SPT;
+    const int p = blockIdx.y;
+    const int j0 = p * j_chunk;
+    const int j1 = min(a.n_src, j0 + j_chunk);
+
+    int idx[SPT];   // own index for sinks that are also sources, else -1
+    double px[SPT], py[SPT], pz[SPT], ax[SPT], ay[SPT], az[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; ++s) {
+        const int i = i0 + s * BLOCK + tid;
+        idx[s] = i < a.n_self ? i : -1;
+        const int ic = i < a.n_sink ? i : a.n_sink - 1;
+        px[s] = a.px[ic]; py[s] = a.py[ic]; pz[s] = a.pz[ic];
+        ax[s] = 0.0; ay[s] = 0.0; az[s] = 0.0;
+    }
+    // MIXED: blocks holding any massive sink evaluate them all in FP64.
+    const bool block_fp64 = (MODE == ABIE_FORCE_MIXED) && (i0 < a.n_self);
+    // sinks of this block that are also sources lie in [i0, i_self_end)
+    const int i_self_end = min(i0 + BLOCK * SPT, a.n_self);
+    const Coef kc = make_coef();
+
+    for (int k0 = j0; k0 < j1; k0 += BLOCK) {
+        const int cnt = min(BLOCK, j1 - k0);
+        __syncthreads();
+        if (tid < cnt) {
+            tx[tid] = a.sx[k0 + tid]; ty[tid] = a.sy[k0 + tid];
+            tz[tid] = a.sz[k0 + tid]; tg[tid] = a.gm[k0 + tid];
+        }
+        __syncthreads();
+        const bool diag = (i0 < a.n_self) && (k0 < i_self_end) && (k0 + cnt > i0);
+        if (MODE == ABIE_FORCE_MIXED) {
+            if (block_fp64) {
+                if (diag) tile_sum<SPT, ABIE_FORCE_FP64, true>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+                else tile_sum<SPT, ABIE_FORCE_FP64, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+            } else {
+                tile_sum<SPT, ABIE_FORCE_MIXED, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+            }
+        } else {
+            if (diag) tile_sum<SPT, MODE, true>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+            else tile_sum<SPT, MODE, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SPT; ++s) {
+        const int i = i0 + s * BLOCK + tid;
+        if (i < a.n_sink) {
+            if (OUT_PARTIAL) {
+                const size_t base = (size_t)p * 3 * a.n_sink;
+                part[base + i] = ax[s];
+                part[base + a.n_sink + i] = ay[s];
+                part[base + 2 * (size_t)a.n_sink + i] = az[s];
+            } else {
+                a.ax[i] = ax[s]; a.ay[i] = ay[s]; a.az[i] = az[s];
+            }
+        }
+    }
+}
+
+// Sum the j-split partials in part order 0..P-1 (deterministic).
+__global__ __launch_bounds__(BLOCK) void reduce_partials(const double *__restrict__ part, int P, int n,
+                                                         double *__restrict__ ax, double *__restrict__ ay,
+                                                         double *__restrict__ az) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double sx = part[i], sy = part[n + i], sz = part[2 * (size_t)n + i];
+    for (int p = 1; p < P; ++p) {
+        const size_t b = (size_t)p * 3 * n;
+        sx += part[b + i]; sy += part[b + n + i]; sz += part[b + 2 * (size_t)n + i];
+    }
+    ax[i] = sx; ay[i] = sy; az[i] = sz;
+}
+
+template <int SPT, int MODE>
+static hipError_t launch_mode(const AccelArgs &a, const AccelPlan &pl, double *part, hipStream_t st) {
+    const int nblk = (a.n_sink + BLOCK * SPT - 1) / (BLOCK * SPT);
+    if (pl.parts <= 1) {
+        hipLaunchKernelGGL((accel_kernel<SPT, MODE, false>), dim3(nblk, 1), dim3(BLOCK), 0, st, a, a.n_src,
+                           nullptr);
+    } else {
+        const int chunk = (a.n_src + pl.parts - 1) / pl.parts;
+        hipLaunchKernelGGL((accel_kernel<SPT, MODE, true>), dim3(nblk, pl.parts), dim3(BLOCK), 0, st, a, chunk,
+                           part);
+        hipLaunchKernelGGL(reduce_partials, dim3((a.n_sink + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part,
+                           pl.parts, a.n_sink, a.ax, a.ay, a.az);
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_spt(const AccelArgs &a, const AccelPlan &pl, double *part, hipStream_t st) {
+    switch (pl.spt) {
+        case 1: return launch_mode<1, MODE>(a, pl, part, st);
+        case 2: return launch_mode<2, MODE>(a, pl, part, st);
+        default: return launch_mode<4, MODE>(a, pl, part, st);
+    }
+}
+
+AccelPlan plan_accel(int n_src, int n_sink, int mode, int n_cu) {
+    AccelPlan pl;
+    const int want = 4 * n_cu;  // blocks for ~4 resident blocks per CU
+    pl.spt = 4;
+    while (pl.spt > 1 && (n_sink + BLOCK * pl.spt - 1) / (BLOCK * pl.spt) < want) pl.spt /= 2;
+    const int nblk = (n_sink + BLOCK * pl.spt - 1) / (BLOCK * pl.spt);
+    pl.parts = 1;
+    // split the source range when the sinks alone cannot fill the chip
+    if (mode != ABIE_FORCE_FP64_EXACT && nblk < want && n_src >= 8 * BLOCK) {
+        pl.parts = (want + nblk - 1) / nblk;
+        pl.parts = min(pl.parts, n_src / (4 * BLOCK));
+        if (pl.parts < 2) pl.parts = 1;
+    }
+    return pl;
+}
+
+hipError_t launch_accel(const AccelArgs &a, const AccelPlan &pl, double *part, hipStream_t st) {
+    if (a.n_sink <= 0) return hipSuccess;
+    switch (a.mode) {
+        case ABIE_FORCE_FP64: return launch_spt<ABIE_FORCE_FP64>(a, pl, part, st);
+        case ABIE_FORCE_FP64_EXACT: return launch_spt<ABIE_FORCE_FP64_EXACT>(a, pl, part, st);
+        case ABIE_FORCE_MIXED: return launch_spt<ABIE_FORCE_MIXED>(a, pl, part, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-AoS path (calculate_accelerations): gather bodies into SoA with the
+// massive ones first, then scatter accelerations back to AoS.
+__global__ void pack_bodies(const double *__restrict__ pos, const double *__restrict__ m,
+                            const int *__restrict__ perm, int n, int n_src, double G, double *__restrict__ x,
+                            double *__restrict__ y, double *__restrict__ z, double *__restrict__ gm) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = perm[p];
+    x[p] = pos[3 * i]; y[p] = pos[3 * i + 1]; z[p] = pos[3 * i + 2];
+    if (p < n_src) gm[p] = G * m[i];
+}
+
+__global__ void unpack_acc(const int *__restrict__ perm, const double *__restrict__ m, int n,
+                           const double *__restrict__ ax, const double *__restrict__ ay,
+                           const double *__restrict__ az, const double *__restrict__ ext,
+                           double *__restrict__ acc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = perm[p];
+    if (m[i] < 0.0) return;  // deleted body: the caller's value stays (common.c:49-52)
+    double vx = ax[p], vy = ay[p], vz = az[p];
+    if (ext) {
+        vx += ext[3 * i]; vy += ext[3 * i + 1]; vz += ext[3 * i + 2];
+    }
+    acc[3 * i] = vx; acc[3 * i + 1] = vy; acc[3 * i + 2] = vz;
+}
+
+hipError_t launch_pack(const double *pos, const double *m, const int *perm, int n, int n_src, double G,
+                       double *x, double *y, double *z, double *gm, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_bodies, dim3((n + 255) / 256), dim3(256), 0, st, pos, m, perm, n, n_src, G, x, y, z, gm);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const int *perm, const double *m, int n, const double *ax, const double *ay,
+                         const double *az, const double *ext, double *acc, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(unpack_acc, dim3((n + 255) / 256), dim3(256), 0, st, perm, m, n, ax, ay, az, ext, acc);
+    return hipGetLastError();
+}
+
+}  // namespace abie

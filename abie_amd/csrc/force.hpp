@@ -1,0 +1,32 @@
+// force.hpp -- launch interface of the all-pairs force kernels (force.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "abie_hip.h"
+
+namespace abie {
+
+struct AccelArgs {
+    const double *sx, *sy, *sz, *gm;  // sources, SoA, gm = G*m
+    const double *px, *py, *pz;       // sinks, SoA
+    double *ax, *ay, *az;             // accelerations, SoA
+    int n_src, n_sink, n_self;        // sinks [0, n_self) are sources [0, n_self)
+    int mode;                         // ABIE_FORCE_*
+};
+
+struct AccelPlan {
+    int spt = 4;    // sinks per thread
+    int parts = 1;  // source-range split (partials summed in order)
+};
+
+AccelPlan plan_accel(int n_src, int n_sink, int mode, int n_cu);
+// `part` must hold parts*3*n_sink doubles when plan.parts > 1.
+hipError_t launch_accel(const AccelArgs &a, const AccelPlan &plan, double *part, hipStream_t st);
+
+hipError_t launch_pack(const double *pos, const double *m, const int *perm, int n, int n_src, double G,
+                       double *x, double *y, double *z, double *gm, hipStream_t st);
+hipError_t launch_unpack(const int *perm, const double *m, int n, const double *ax, const double *ay,
+                         const double *az, const double *ext, double *acc, hipStream_t st);
+
+}  // namespace abie

@@ -1,0 +1,138 @@
+// gr15.hip -- configuration table, dispatch, energy and event kernels of
+// the GR15 engine (the engine itself is gr15_impl.hpp).
+#include "devmath.hpp"
+#include "gr15.hpp"
+
+namespace abie {
+
+const GR15Cfg kCfgs[kNumCfg] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {512, 2}};
+
+int pick_cfg(int n, int nm) {
+    for (int c = 0; c < kNumCfg; ++c)
+        if (n <= kCfgs[c].nt * kCfgs[c].bpt && nm <= kCfgs[c].nt) return c;
+    return -1;
+}
+
+// LDS: sources [2][NT] x {x, y, z, gm}, then 128 doubles of scratch:
+// [0, 32) reductions, [40, 48) broadcast flags, [64, 128) proposer hidden layer.
+size_t gr15_lds_bytes(int cfg) { return (size_t)2 * kCfgs[cfg].nt * 4 * sizeof(double) + 128 * sizeof(double); }
+
+
+hipError_t launch_gr15_fp64(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
+hipError_t launch_gr15_exact(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
+hipError_t launch_gr15_mixed(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
+
+hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st) {
+    if (nb <= 0) return hipSuccess;
+    if (cfg < 0 || cfg >= kNumCfg) return hipErrorInvalidValue;
+    const size_t lds = gr15_lds_bytes(cfg);
+    switch (fm) {
+        case ABIE_FORCE_FP64: return launch_gr15_fp64(cfg, ctrl, a, nb, lds, st);
+        case ABIE_FORCE_FP64_EXACT: return launch_gr15_exact(cfg, ctrl, a, nb, lds, st);
+        case ABIE_FORCE_MIXED: return launch_gr15_mixed(cfg, ctrl, a, nb, lds, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// check_collisions_close_encounters (common.c:116-158) for one system: all
+// threads filter the pairs with a test that can only over-report; if any
+// pair passes, thread 0 redoes the reference's ordered (j, k) loop exactly
+// and updates the counters (cnt[0] = close encounters, cnt[1] = collisions)
+// and the rings.  *ret receives 0, 1 or 2.
+__global__ __launch_bounds__(256) void events_kernel(const double *__restrict__ pos, const double *__restrict__ rad,
+                                                     int n, double t, double cd, long long *cnt, int max_ce,
+                                                     int max_coll, double *ring_ce, double *ring_coll, int *ret) {
+    __shared__ int hit;
+    if (threadIdx.x == 0) hit = 0;
+    __syncthreads();
+    const double cd2 = cd * cd * (1.0 + 0x1p-50);
+    int mine = 0;
+    for (int j = threadIdx.x; j < n; j += 256)
+        for (int k = j + 1; k < n; ++k) {
+            const double dx = pos[3 * j] - pos[3 * k], dy = pos[3 * j + 1] - pos[3 * k + 1],
+                         dz = pos[3 * j + 2] - pos[3 * k + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double rr = rad[j] + rad[k];
+            mine |= (r2 <= cd2) || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50));
+        }
+    if (mine) atomicOr(&hit, 1);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    long long nce = cnt[0], nco = cnt[1];
+    if (hit) {
+        for (int j = 0; j < n; ++j)
+            for (int k = j + 1; k < n; ++k) {
+                const double dx = pos[3 * j] - pos[3 * k], dy = pos[3 * j + 1] - pos[3 * k + 1],
+                             dz = pos[3 * j + 2] - pos[3 * k + 2];
+                const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                const double rr = rad[j] + rad[k];
+                if (d <= cd) {
+                    if (max_ce > 0) {
+                        const long long w = 4LL * max_ce, o = 4 * nce;
+                        ring_ce[o % w] = t; ring_ce[(o + 1) % w] = j; ring_ce[(o + 2) % w] = k; ring_ce[(o + 3) % w] = d;
+                    }
+                    nce += 1;
+                }
+                if (rr > 0 && d <= rr) {
+                    if (max_coll > 0) {
+                        const long long w = 4LL * max_coll, o = 4 * nco;
+                        ring_coll[o % w] = t; ring_coll[(o + 1) % w] = j; ring_coll[(o + 2) % w] = k;
+                        ring_coll[(o + 3) % w] = d;
+                    }
+                    nco += 1;
+                }
+            }
+    }
+    cnt[0] = nce;
+    cnt[1] = nco;
+    *ret = (max_ce > 0 && nce >= max_ce) ? 1 : (max_coll > 0 && nco >= max_coll) ? 2 : 0;
+}
+
+hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
+                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *ret, hipStream_t st) {
+    hipLaunchKernelGGL(events_kernel, dim3(1), dim3(256), 0, st, pos, rad, n, t, cd, cnt, max_ce, max_coll, ring_ce,
+                       ring_coll, ret);
+    return hipGetLastError();
+}
+
+// calculate_energy (common.c:340-357) per system: one block per system; each
+// thread forms body i's term (kinetic, then its pair terms in ascending j),
+// and the block sums the per-body terms in a fixed tree order.
+__global__ __launch_bounds__(256) void energy_kernel(const SysDesc *sys, const double *x, const double *y,
+                                                     const double *z, const double *vx, const double *vy,
+                                                     const double *vz, const double *m, double G, double *out) {
+    __shared__ double part[256];
+    const SysDesc d = sys[blockIdx.x];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < d.n; i += 256) {
+        const int gi = d.body0 + i;
+        const double mi = m[gi];
+        if (mi == 0) continue;
+        const double v = sqrt(vx[gi] * vx[gi] + vy[gi] * vy[gi] + vz[gi] * vz[gi]);
+        double e = 0.5 * mi * (v * v);
+        for (int j = 0; j < d.n; ++j) {
+            const int gj = d.body0 + j;
+            if (j == i || m[gj] == 0) continue;
+            const double dx = x[gi] - x[gj], dy = y[gi] - y[gj], dz = z[gi] - z[gj];
+            e -= 0.5 * G * mi * m[gj] / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        acc += e;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,
+                         const double *vx, const double *vy, const double *vz, const double *m, double G,
+                         double *energy, hipStream_t st) {
+    if (n_sys <= 0) return hipSuccess;
+    hipLaunchKernelGGL(energy_kernel, dim3(n_sys), dim3(256), 0, st, sys, x, y, z, vx, vy, vz, m, G, energy);
+    return hipGetLastError();
+}
+
+}  // namespace abie

@@ -1,0 +1,80 @@
+// gr15.hpp -- the batched, device-resident Gauss-Radau15 engine (gr15.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "abie_hip.h"
+
+namespace abie {
+
+// One independent system: bodies [body0, body0 + n) of the flat arrays, in
+// engine order (massive bodies first, ascending original index, then the
+// rest, ascending).  nm = number of massive bodies (m > 0).
+struct SysDesc {
+    int body0, n, nm, cfg;
+};
+
+// Per-system control block, device-resident; persists across launches.
+struct SysCtl {
+    double t, h, t_end;
+    int status;   // ST_*
+    int phase;    // 0: needs the initial step, 1: stepping
+    int rejects;  // consecutive rejections (step_loop_count)
+    int warned;
+    long long n_evals, n_attempts, n_accepted, n_pc, n_prop_used, n_prop_rej;
+    long long n_ce, n_coll;  // event counters (sim.n_close_encounters, ...)
+    double prev_h, prev_err, prev_dtreq;  // history used by the step proposer
+    int prev_pc, pad;
+};
+
+enum : int { ST_RUNNING = 0, ST_CE = 1, ST_COLL = 2, ST_DONE = 3, ST_ERROR = -1 };
+
+// Number of persistent doubles per body: b[7][3], g[7][3], a0[3], b6_prev[3].
+constexpr int kStateDoubles = 48;
+
+struct GR15Args {
+    const SysDesc *sys;
+    const int *order;  // block -> system index
+    SysCtl *ctl;
+    // flat body arrays (engine order)
+    double *x, *y, *z, *vx, *vy, *vz;  // positions / velocities, updated in place
+    const double *gm;                  // G*m for massive bodies, else 0
+    const double *rad;                 // radii
+    const double *ext;                 // [3][total] constant extra acceleration, or null
+    const int *orig;                   // original index within the system
+    const int *inv;                    // engine index of original index (per system)
+    const unsigned char *deleted;      // m < 0
+    double *state;                     // [kStateDoubles][total]
+    double *scan;                      // [3][total] event-scan positions
+    long long total;
+    // events (per system rings, 4 doubles per event)
+    double ce_dist;
+    int max_ce, max_coll;
+    double *ring_ce, *ring_coll;  // [n_sys][4*max]
+    // step log
+    double *log;
+    unsigned long long *log_len;
+    long long log_cap;
+    // learned proposer (FP32 weights) or null
+    const float *mlp;
+    int max_attempts;  // per launch
+    int events;        // 0: skip the close-encounter / collision scan
+};
+
+// Engine configurations: threads per block and bodies per thread.
+struct GR15Cfg {
+    int nt, bpt;
+};
+constexpr int kNumCfg = 5;
+extern const GR15Cfg kCfgs[kNumCfg];
+int pick_cfg(int n, int nm);  // -1 if no configuration fits
+size_t gr15_lds_bytes(int cfg);
+
+hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st);
+hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
+                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *ret, hipStream_t st);
+hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,
+                         const double *vx, const double *vy, const double *vz, const double *m, double G,
+                         double *energy, hipStream_t st);
+
+}  // namespace abie

@@ -1,0 +1,9 @@
+// gr15_exact.hip -- GR15 engine instantiated for ABIE_FORCE_FP64_EXACT forces (one
+// translation unit per force mode so the three compile in parallel).
+#include "gr15_impl.hpp"
+
+namespace abie {
+hipError_t launch_gr15_exact(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    return launch_gr15_fm<ABIE_FORCE_FP64_EXACT>(cfg, ctrl, a, nb, lds, st);
+}
+}  // namespace abie

@@ -1,0 +1,749 @@
+// gr15_impl.hpp -- batched, device-resident Gauss-Radau15 (Everhart) integrator.
+//
+// Restates src/integrator_gauss_radau15.c:9-380 for MI355X: one workgroup
+// integrates one system end to end inside the kernel, so a GR15 attempt costs
+// no host round trip.  Each thread owns BPT bodies (body b = k*NT + tid) and
+// keeps their whole integrator state in registers: y0, dy0, a0 (=ddy0),
+// b[7], g[7] and the previous b[6] (the only column of bs0 ever read), 54
+// doubles per body.  Massive bodies come first in engine order, so they sit
+// in slot 0 of the first waves; at every force evaluation they write their
+// predicted positions to an LDS source list (double-buffered, one barrier per
+// evaluation) and every thread sums the sources for its bodies from LDS.
+//
+// Bit-exactness with the reference C (gcc -O3 -std=c99, no contraction):
+//  * the library is compiled with -ffp-contract=off and every expression of
+//    approx_pos/approx_vel/compute_gs/compute_bs_from_gs/refine_bs keeps the
+//    reference's operation order;
+//  * divisions by the predictor constants use div_const (correctly rounded);
+//  * node 0 reuses a0 instead of a new evaluation: the reference evaluates
+//    the force at y(s=0) = y0 again and gets a0 bit for bit;
+//  * compute_gs rebuilds all g[0..ih-1] at node ih from unchanged inputs, so
+//    only g[ih-1] can differ from the previous node's values: only it is
+//    recomputed, by the same recurrence;
+//  * vector_max_abs ignores NaNs and starts at -DBL_MAX: fmax reductions do
+//    the same; the exponents 1/7 and 1/16 go through pow_pos (double-double).
+//  With ABIE_FORCE_FP64_EXACT forces the whole trajectory matches the
+//  reference bit for bit (tests/test_gpu_gr15.py).
+#pragma once
+
+#ifndef ABIE_GR15_WPE
+#define ABIE_GR15_WPE 2  // waves per SIMD the engine is compiled for (256 VGPRs)
+#endif
+
+#include "devmath.hpp"
+#include "gr15.hpp"
+#include "radau_consts.h"
+
+namespace abie {
+
+constexpr double kH[8] = ABIE_RADAU_H;
+constexpr double kR[8][8] = ABIE_RADAU_RS;
+constexpr double kC[8][8] = ABIE_RADAU_CS;
+// integrator_gauss_radau15.h:17,58-63
+constexpr double kEpsb = 1.e-4, kFac = 0.25, kTolpc = 1.e-16, kHmin = 1.e-13;
+
+// Every FP64 constant of the step is read from this table through a pointer
+// the compiler cannot see through (launder()), so each use is a scalar load
+// into SGPRs instead of a value hoisted into VGPRs for the whole kernel
+// (gfx950 VOP3 takes no literal; ~80 hoisted constants would cost ~160 VGPRs).
+struct KTab {
+    double R[8][8], C[8][8];
+    double pc[8], prc[8];   // approx_pos divisors 3,6,10,15,21,28,36 and 1/them
+    double vc[8], vrc[8];   // approx_vel divisors 2..8 and 1/them
+    double h[8];
+    double ref[8];          // 7, 6, 5, 4, 3, 2, 21, 15
+    double ref2[8];         // 10, 35, 20, 1.5, 1.875, 0.5, 0.25, 0.125
+};
+
+constexpr KTab make_ktab() {
+    KTab k{};
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+            k.R[i][j] = kR[i][j];
+            k.C[i][j] = kC[i][j];
+        }
+    const double pc[7] = {3, 6, 10, 15, 21, 28, 36}, vc[7] = {2, 3, 4, 5, 6, 7, 8};
+    for (int i = 0; i < 7; ++i) {
+        k.pc[i] = pc[i];
+        k.prc[i] = 1.0 / pc[i];
+        k.vc[i] = vc[i];
+        k.vrc[i] = 1.0 / vc[i];
+    }
+    for (int i = 0; i < 8; ++i) k.h[i] = kH[i];
+    const double r1[8] = {7, 6, 5, 4, 3, 2, 21, 15}, r2[8] = {10, 35, 20, 1.5, 1.875, 0.5, 0.25, 0.125};
+    for (int i = 0; i < 8; ++i) {
+        k.ref[i] = r1[i];
+        k.ref2[i] = r2[i];
+    }
+    return k;
+}
+
+static __constant__ KTab gK = make_ktab();
+
+__device__ __forceinline__ const KTab *launder() {
+    const KTab *p = &gK;
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// -------------------------------------------------------------- predictors
+// approx_pos (integrator_gauss_radau15.c:56-61) at node s, T = dt*s.
+__device__ __forceinline__ double pred_pos(const KTab *K, double y0, double dy0, double a0, const double (&b)[7],
+                                           double s, double T) {
+    double p = div_const(b[5], K->pc[5], K->prc[5]) + div_const(s * b[6], K->pc[6], K->prc[6]);
+    p = div_const(b[4], K->pc[4], K->prc[4]) + s * p;
+    p = div_const(b[3], K->pc[3], K->prc[3]) + s * p;
+    p = div_const(b[2], K->pc[2], K->prc[2]) + s * p;
+    p = div_const(b[1], K->pc[1], K->prc[1]) + s * p;
+    p = div_const(b[0], K->pc[0], K->prc[0]) + s * p;
+    p = a0 + s * p;
+    return y0 + T * (dy0 + (T * p) * K->ref2[5]);
+}
+
+// approx_vel (integrator_gauss_radau15.c:63-68) at node s = 1.
+__device__ __forceinline__ double pred_vel1(const KTab *K, double dy0, double a0, const double (&b)[7], double dt) {
+    double p = div_const(b[5], K->vc[5], K->vrc[5]) + b[6] * K->ref2[7];
+    p = div_const(b[4], K->vc[4], K->vrc[4]) + p;
+    p = div_const(b[3], K->vc[3], K->vrc[3]) + p;
+    p = b[2] * K->ref2[6] + p;
+    p = div_const(b[1], K->vc[1], K->vrc[1]) + p;
+    p = b[0] * K->ref2[5] + p;
+    p = a0 + p;
+    return dy0 + dt * p;
+}
+
+// compute_gs + compute_bs_from_gs at node IH (1..7) for one component.
+template <int IH>
+__device__ __forceinline__ void update_gb(const KTab *K, double F, double a0, double (&g)[7], double (&b)[7]) {
+    constexpr int k = IH - 1;
+    double v = (F - a0) * K->R[k + 1][0];
+#pragma unroll
+    for (int l = 1; l <= k; ++l) v = (v - g[l - 1]) * K->R[k + 1][l];
+    g[k] = v;
+#pragma unroll
+    for (int kk = 0; kk < IH; ++kk) {
+        double s = g[kk];  // cs[kk][kk] == 1 exactly
+#pragma unroll
+        for (int j = kk + 1; j < 7; ++j) s = s + K->C[j][kk] * g[j];
+        b[kk] = s;
+    }
+}
+
+// refine_bs (integrator_gauss_radau15.c:178-218) for one component.
+__device__ __forceinline__ void refine(const KTab *K, double (&b)[7], double q) {
+    const double q2 = q * q, q3 = q2 * q, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q2 * q5;
+    const double c7 = K->ref[0], c6 = K->ref[1], c5 = K->ref[2], c4 = K->ref[3], c3 = K->ref[4], c2 = K->ref[5];
+    const double c21 = K->ref[6], c15 = K->ref[7], c10 = K->ref2[0], c35 = K->ref2[1], c20 = K->ref2[2];
+    double e[7];
+    e[0] = q * (b[6] * c7 + b[5] * c6 + b[4] * c5 + b[3] * c4 + b[2] * c3 + b[1] * c2 + b[0]);
+    e[1] = q2 * (b[6] * c21 + b[5] * c15 + b[4] * c10 + b[3] * c6 + b[2] * c3 + b[1]);
+    e[2] = q3 * (b[6] * c35 + b[5] * c20 + b[4] * c10 + b[3] * c4 + b[2]);
+    e[3] = q4 * (b[6] * c35 + b[5] * c15 + b[4] * c5 + b[3]);
+    e[4] = q5 * (b[6] * c21 + b[5] * c6 + b[4]);
+    e[5] = q6 * (b[6] * c7 + b[5]);
+    e[6] = q7 * b[6];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) b[j] = e[j] + 0.0;
+}
+
+// -------------------------------------------------------------- pair terms
+struct Coef2 {
+    double c1, c2;
+};
+
+template <int FM>
+__device__ __forceinline__ void pair(double px, double py, double pz, const double *s, double &ax, double &ay,
+                                     double &az, Coef2 k) {
+    const double dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+    if (FM == ABIE_FORCE_FP64_EXACT) {
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r3 = sqrt(r2) * r2;
+        ax = ax - s[3] * dx / r3;
+        ay = ay - s[3] * dy / r3;
+        az = az - s[3] * dz / r3;
+    } else if (FM == ABIE_FORCE_FP64) {
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double y = __builtin_amdgcn_rsq(r2);
+        const double y2 = y * y;
+        const double e = fma(-r2, y2, 1.0);
+        const double g3 = (s[3] * y) * y2;
+        const double t = fma(g3 * e, fma(e, k.c2, k.c1), g3);
+        ax = fma(-t, dx, ax);
+        ay = fma(-t, dy, ay);
+        az = fma(-t, dz, az);
+    } else {  // MIXED: FP32 r^-3 from FP64 relative coordinates
+        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+        const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+        float y = __builtin_amdgcn_rsqf(r2);
+        const float e = fmaf(-r2 * y, y, 1.0f);
+        y = fmaf(0.5f * y, e, y);
+        const float t = ((float)s[3] * y) * (y * y);
+        ax = ax - (double)(t * fx);
+        ay = ay - (double)(t * fy);
+        az = az - (double)(t * fz);
+    }
+}
+
+// The same pair with the self-pair (source index == own) removed.
+template <int FM>
+__device__ __forceinline__ void pair_self(double px, double py, double pz, const double *s, bool self, double &ax,
+                                          double &ay, double &az, Coef2 k) {
+    const double dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+    if (FM == ABIE_FORCE_FP64_EXACT) {
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r3 = sqrt(r2) * r2;
+        const double tx = s[3] * dx / r3, ty = s[3] * dy / r3, tz = s[3] * dz / r3;
+        ax = ax - (self ? 0.0 : tx);
+        ay = ay - (self ? 0.0 : ty);
+        az = az - (self ? 0.0 : tz);
+    } else {  // massive sinks always use FP64
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double y = __builtin_amdgcn_rsq(r2);
+        const double y2 = y * y;
+        const double e = fma(-r2, y2, 1.0);
+        const double g3 = (s[3] * y) * y2;
+        double t = fma(g3 * e, fma(e, k.c2, k.c1), g3);
+        t = self ? 0.0 : t;
+        ax = fma(-t, dx, ax);
+        ay = fma(-t, dy, ay);
+        az = fma(-t, dz, az);
+    }
+}
+
+// -------------------------------------------------------------- reductions
+// Workgroup max of two values; NaNs lose (fmax), as in vector_max_abs.
+template <int NT>
+__device__ __forceinline__ void wg_max2(double &a, double &b, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off));
+        b = fmax(b, __shfl_xor(b, off));
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = a;
+        red[2 * w + 1] = b;
+    }
+    __syncthreads();
+    a = red[0];
+    b = red[1];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) {
+        a = fmax(a, red[2 * i]);
+        b = fmax(b, red[2 * i + 1]);
+    }
+}
+
+// -------------------------------------------------------------- the kernel
+template <int NT, int BPT>
+struct Bodies {
+    double y0[BPT][3], dy0[BPT][3], a0[BPT][3], bb[BPT][3][7], gg[BPT][3][7], b6p[BPT][3];
+};
+
+template <int NT, int BPT, int FM, int CT>
+struct Engine {
+    const GR15Args &A;
+    const int sys, body0, n, nm;
+    double *src;  // LDS [2][NT][4]
+    double *red;  // LDS [32]
+    int buf = 0;
+    Coef2 kc;
+    int slot_valid[BPT], slot_massive[BPT], slot_del[BPT], slot_orig[BPT];
+    bool wave_massive;
+
+    __device__ Engine(const GR15Args &a, int s, const SysDesc &d, double *lds)
+        : A(a), sys(s), body0(d.body0), n(d.n), nm(d.nm), src(lds), red(lds + 2 * NT * 4) {
+        kc = Coef2{1.5, 1.875};
+        asm volatile("" : "+v"(kc.c1), "+v"(kc.c2));
+        wave_massive = ((int)(threadIdx.x & ~63) < nm);
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const int b = k * NT + threadIdx.x;
+            slot_valid[k] = b < n;
+            slot_massive[k] = b < nm;
+            slot_del[k] = (b < n) ? (int)A.deleted[body0 + b] : 0;
+            slot_orig[k] = (b < n) ? A.orig[body0 + b] : 0;
+        }
+    }
+
+    // accelerations at positions p for every owned body; one barrier
+    __device__ void forces(const double (&p)[BPT][3], double (&acc)[BPT][3]) {
+        double *S = src + buf * NT * 4;
+        buf ^= 1;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const int b = k * NT + threadIdx.x;
+            if (b < nm) {
+                S[4 * b] = p[k][0];
+                S[4 * b + 1] = p[k][1];
+                S[4 * b + 2] = p[k][2];
+                S[4 * b + 3] = A.gm[body0 + b];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+        if (wave_massive) {
+            // slot 0 holds massive bodies (nm <= NT): FP64 with the self-pair removed
+            const int me = threadIdx.x;
+            for (int j = 0; j < nm; ++j) {
+                const double *s = S + 4 * j;
+                pair_self<FM>(p[0][0], p[0][1], p[0][2], s, j == me, acc[0][0], acc[0][1], acc[0][2], kc);
+#pragma unroll
+                for (int k = 1; k < BPT; ++k) pair<FM>(p[k][0], p[k][1], p[k][2], s, acc[k][0], acc[k][1], acc[k][2], kc);
+            }
+        } else {
+#pragma unroll 2
+            for (int j = 0; j < nm; ++j) {
+                const double *s = S + 4 * j;
+#pragma unroll
+                for (int k = 0; k < BPT; ++k) pair<FM>(p[k][0], p[k][1], p[k][2], s, acc[k][0], acc[k][1], acc[k][2], kc);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const int b = k * NT + threadIdx.x;
+            if (slot_del[k]) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+            if (A.ext != nullptr && b < n) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[k][c] += A.ext[c * A.total + body0 + b];
+            }
+        }
+    }
+
+    // which components enter the controller's norms
+    __device__ __forceinline__ bool in_norm(int k) const {
+        return CT == ABIE_CTRL_PLAIN ? (bool)slot_valid[k] : (bool)slot_massive[k];
+    }
+};
+
+// Features of the learned step proposer (ABIE_MLP_IN values), computed by
+// every thread from LDS-broadcast system scalars.
+struct PropFeat {
+    float f[ABIE_MLP_IN];
+};
+
+__device__ float mlp_eval(const float *__restrict__ w, const float *x, float *lds_h) {
+    // 12 -> 64 (ReLU) -> 64 (ReLU) -> 1; one wave, lane = hidden unit
+    const int lane = threadIdx.x & 63;
+    const float *W1 = w, *b1 = W1 + ABIE_MLP_IN * ABIE_MLP_HIDDEN;
+    const float *W2 = b1 + ABIE_MLP_HIDDEN, *b2 = W2 + ABIE_MLP_HIDDEN * ABIE_MLP_HIDDEN;
+    const float *w3 = b2 + ABIE_MLP_HIDDEN, *b3 = w3 + ABIE_MLP_HIDDEN;
+    float h = b1[lane];
+    for (int i = 0; i < ABIE_MLP_IN; ++i) h = fmaf(x[i], W1[i * ABIE_MLP_HIDDEN + lane], h);
+    h = fmaxf(h, 0.0f);
+    lds_h[lane] = h;
+    __builtin_amdgcn_wave_barrier();
+    float h2 = b2[lane];
+    for (int i = 0; i < ABIE_MLP_HIDDEN; ++i) h2 = fmaf(lds_h[i], W2[i * ABIE_MLP_HIDDEN + lane], h2);
+    h2 = fmaxf(h2, 0.0f);
+    float o = h2 * w3[lane];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
+    return o + b3[0];
+}
+
+template <int NT, int BPT, int FM, int CT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WPE))) void gr15_kernel(GR15Args A) {
+    extern __shared__ double lds[];
+    const int sys = A.order[blockIdx.x];
+    const SysDesc d = A.sys[sys];
+    SysCtl *ctl = A.ctl + sys;
+    if (ctl->status != ST_RUNNING) return;
+
+    Engine<NT, BPT, FM, CT> E(A, sys, d, lds);
+    const int tid = threadIdx.x;
+    const long long T = A.total;
+    const int b0 = d.body0, n = d.n, nm = d.nm;
+    double *flag = E.red + 40;  // LDS scalars broadcast by thread 0
+
+    double y0[BPT][3], dy0[BPT][3], a0[BPT][3], bb[BPT][3][7], gg[BPT][3][7], b6p[BPT][3];
+    double yp[BPT][3], acc[BPT][3];
+
+    // ---- load state
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        const int b = k * NT + tid;
+        const long long gi = b0 + (b < n ? b : 0);
+        y0[k][0] = A.x[gi]; y0[k][1] = A.y[gi]; y0[k][2] = A.z[gi];
+        dy0[k][0] = A.vx[gi]; dy0[k][1] = A.vy[gi]; dy0[k][2] = A.vz[gi];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                bb[k][c][j] = A.state[(long long)(c * 7 + j) * T + gi];
+                gg[k][c][j] = A.state[(long long)(21 + c * 7 + j) * T + gi];
+            }
+            a0[k][c] = A.state[(long long)(42 + c) * T + gi];
+            b6p[k][c] = A.state[(long long)(45 + c) * T + gi];
+        }
+    }
+    double t = ctl->t, h = ctl->h;
+    const double t_end = ctl->t_end;
+    int rejects = ctl->rejects, warned = ctl->warned;
+    long long n_evals = 0, n_att = 0, n_acc = 0, n_pc = 0, n_pu = 0, n_pr = 0;
+    int status = ST_RUNNING;
+    int prop_pending = ctl->pad;  // the current h came from the proposer
+
+    // ---- fresh start: a0 = a(y0), initial_time_step (integrator_gauss_radau15.c:9-54,270-273)
+    if (ctl->phase == 0) {
+        E.forces(y0, a0);
+        n_evals += 1;
+        // d0, d1 over the first n of the 3n components (vector_max_abs(y0, nbodies))
+        double d0 = -DBL_MAX, d1 = -DBL_MAX;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n) {
+                    d0 = fmax(d0, fabs(y0[k][c]));
+                    d1 = fmax(d1, fabs(a0[k][c]));
+                }
+        wg_max2<NT>(d0, d1, E.red);
+        const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) yp[k][c] = y0[k][c] + dt0 * dy0[k][c];
+        E.forces(yp, acc);
+        n_evals += 1;
+        double d2 = -DBL_MAX, dummy = -DBL_MAX;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (E.slot_valid[k]) d2 = fmax(d2, fabs(acc[k][c] - a0[k][c]));
+        wg_max2<NT>(d2, dummy, E.red);
+        d2 = d2 / dt0;
+        double dt1;
+        if (fmax(d1, d2) <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
+        else dt1 = pow_pos(0.01 / fmax(d1, d2), 1.0 / 16);
+        h = fmin(100 * dt0, dt1);
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                b6p[k][c] = 0.0;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) bb[k][c][j] = gg[k][c][j] = 0.0;
+            }
+        rejects = 0;
+        warned = 0;
+    }
+
+    // ---- the step loop (integrator_gauss_radau15.c:279-368)
+    double dy[BPT][3];
+    while (n_att < A.max_attempts) {
+        int pc_used = 0;
+        for (int ipc = 0; ipc < 12; ++ipc) {
+            pc_used = ipc + 1;
+            double mdb = -DBL_MAX, mf = -DBL_MAX;
+#define ABIE_NODE(IH)                                                                          \
+    {                                                                                          \
+        const KTab *K = launder();                                                             \
+        const double s = K->h[IH];                                                             \
+        const double Ts = h * s;                                                               \
+        _Pragma("unroll") for (int k = 0; k < BPT; ++k) _Pragma("unroll") for (int c = 0; c < 3; ++c) \
+            yp[k][c] = pred_pos(K, y0[k][c], dy0[k][c], a0[k][c], bb[k][c], s, Ts);          \
+        E.forces(yp, acc);                                                                     \
+        const KTab *K2 = launder();                                                            \
+        _Pragma("unroll") for (int k = 0; k < BPT; ++k) _Pragma("unroll") for (int c = 0; c < 3; ++c) \
+            update_gb<IH>(K2, acc[k][c], a0[k][c], gg[k][c], bb[k][c]);                       \
+    }
+            ABIE_NODE(1) ABIE_NODE(2) ABIE_NODE(3) ABIE_NODE(4) ABIE_NODE(5) ABIE_NODE(6) ABIE_NODE(7)
+#undef ABIE_NODE
+            n_evals += 7;
+#pragma unroll
+            for (int k = 0; k < BPT; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (E.in_norm(k)) {
+                        mdb = fmax(mdb, fabs(bb[k][c][6] - b6p[k][c]));
+                        mf = fmax(mf, fabs(acc[k][c]));
+                    }
+            wg_max2<NT>(mdb, mf, E.red);
+            if (mdb / mf < kTolpc) break;
+#pragma unroll
+            for (int k = 0; k < BPT; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) b6p[k][c] = bb[k][c][6];
+        }
+        n_pc += pc_used;
+        n_att += 1;
+
+        // advance: predictor at s = 1, final evaluation
+        const KTab *KA = launder();
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                yp[k][c] = pred_pos(KA, y0[k][c], dy0[k][c], a0[k][c], bb[k][c], 1.0, h * 1.0);
+                dy[k][c] = pred_vel1(KA, dy0[k][c], a0[k][c], bb[k][c], h * 1.0);
+            }
+        E.forces(yp, acc);
+        n_evals += 1;
+        double mb = -DBL_MAX, ma = -DBL_MAX;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (E.in_norm(k)) {
+                    mb = fmax(mb, fabs(bb[k][c][6]));
+                    ma = fmax(ma, fabs(acc[k][c]));
+                }
+        wg_max2<NT>(mb, ma, E.red);
+        const double estim = mb / ma;
+        const double err = pow_pos(estim / kEpsb, 1. / 7);
+        const double dtreq = h / err;
+        const double h_att = h, t_att = t;
+        bool accepted = false;
+        bool finished = false;
+        double h_next_plain;
+
+        if (err <= 1 || rejects > 100) {
+            accepted = true;
+            t += h;
+            rejects = 0;
+            if (t > t_end) finished = true;
+#pragma unroll
+            for (int k = 0; k < BPT; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    y0[k][c] = yp[k][c];
+                    dy0[k][c] = dy[k][c];
+                    a0[k][c] = acc[k][c];
+                    b6p[k][c] = bb[k][c][6];
+                }
+            n_acc += 1;
+        } else {
+            rejects += 1;
+        }
+        // step size (integrator_gauss_radau15.c:349-367)
+        if (dtreq / h > 1.0 / kFac) h_next_plain = h / kFac;
+        else if (dtreq < 1.e-12) h_next_plain = h * kFac;
+        else h_next_plain = dtreq;
+        double h_next = h_next_plain;
+        double q = dtreq / h_att;  // refine_bs ratio of the reference
+
+        // the learned proposer (after accepted steps only)
+        if (CT == ABIE_CTRL_LEARNED && accepted && !finished && A.mlp != nullptr) {
+            if (tid < 64) {
+                float x[ABIE_MLP_IN];
+                const double eps = 1e-300;
+                x[0] = (float)log10(h_att);
+                x[1] = (float)log10(err + eps);
+                x[2] = (float)log10(ctl->prev_err + eps);
+                x[3] = (float)log10(h_att / (ctl->prev_h > 0 ? ctl->prev_h : h_att));
+                x[4] = (float)pc_used / 12.0f;
+                x[5] = (float)log10(estim + eps);
+                x[6] = (float)log10((double)nm);
+                x[7] = (float)log10((double)(n - nm) + 1.0);
+                x[8] = (float)log10(dtreq / h_att);
+                x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
+                x[10] = (float)ctl->prev_pc / 12.0f;
+                x[11] = 1.0f;
+                const float yl = mlp_eval(A.mlp, x, (float *)(E.red + 64));
+                if (tid == 0) flag[0] = (double)yl;
+            }
+            __syncthreads();
+            double ratio = exp10(fmin(fmax(flag[0], -1.0), 1.0));
+            double hp = dtreq * ratio;
+            // keep the reference's growth / shrink bounds
+            hp = fmin(hp, h_att / kFac);
+            hp = fmax(hp, h_att * kFac);
+            h_next = hp;
+            q = hp / h_att;
+            n_pu += 1;
+        }
+        if (prop_pending && !accepted) n_pr += 1;
+        prop_pending = (CT == ABIE_CTRL_LEARNED && accepted && !finished && A.mlp != nullptr);
+
+        // step log
+        if (A.log != nullptr && tid == 0) {
+            const unsigned long long slot = atomicAdd(A.log_len, 1ull);
+            if ((long long)slot < A.log_cap) {
+                double *r = A.log + slot * ABIE_STEPLOG_FIELDS;
+                r[0] = sys; r[1] = t_att; r[2] = h_att; r[3] = estim; r[4] = err; r[5] = dtreq;
+                r[6] = accepted; r[7] = pc_used; r[8] = rejects; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
+                r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = n - nm; r[15] = h_next;
+            }
+        }
+        if (accepted) {
+            if (tid == 0) {
+                ctl->prev_h = h_att;
+                ctl->prev_err = err;
+                ctl->prev_dtreq = dtreq;
+                ctl->prev_pc = pc_used;
+            }
+            if (!finished) {
+                const KTab *KR = launder();
+#pragma unroll
+                for (int k = 0; k < BPT; ++k)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) refine(KR, bb[k][c], q);
+            }
+            // close encounters / collisions (common.c:116-158) on the new positions
+            if (A.events) {
+                double *sx = A.scan, *sy = A.scan + T, *sz = A.scan + 2 * T;
+#pragma unroll
+                for (int k = 0; k < BPT; ++k) {
+                    const int b = k * NT + tid;
+                    if (b < n) {
+                        sx[b0 + b] = y0[k][0]; sy[b0 + b] = y0[k][1]; sz[b0 + b] = y0[k][2];
+                    }
+                }
+                __syncthreads();
+                const double cd = A.ce_dist;
+                const double cd2 = cd * cd * (1.0 + 0x1p-50);
+                double hit = 0.0, dummy2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < BPT; ++k) {
+                    const int j = k * NT + tid;
+                    if (j >= n) continue;
+                    const double rj = A.rad[b0 + j];
+                    for (int kk = 0; kk < n; ++kk) {
+                        if (kk == j) continue;
+                        const double dx = y0[k][0] - sx[b0 + kk], dyy = y0[k][1] - sy[b0 + kk],
+                                     dz = y0[k][2] - sz[b0 + kk];
+                        const double r2 = dx * dx + dyy * dyy + dz * dz;
+                        const double rr = rj + A.rad[b0 + kk];
+                        if (r2 <= cd2 || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50))) hit = 1.0;
+                    }
+                }
+                wg_max2<NT>(hit, dummy2, E.red);
+                if (hit > 0.0) {
+                    // exact pass in the reference's (j, k) order, one thread
+                    if (tid == 0) {
+                        long long nce = ctl->n_ce, nco = ctl->n_coll;
+                        double *rce = A.ring_ce ? A.ring_ce + (long long)sys * 4 * A.max_ce : nullptr;
+                        double *rco = A.ring_coll ? A.ring_coll + (long long)sys * 4 * A.max_coll : nullptr;
+                        for (int jo = 0; jo < n; ++jo) {
+                            const int je = A.inv[b0 + jo];
+                            for (int ko = jo + 1; ko < n; ++ko) {
+                                const int ke = A.inv[b0 + ko];
+                                const double dx = sx[b0 + je] - sx[b0 + ke], dyy = sy[b0 + je] - sy[b0 + ke],
+                                             dz = sz[b0 + je] - sz[b0 + ke];
+                                const double r2 = dx * dx + dyy * dyy + dz * dz;
+                                const double dd = sqrt(r2);
+                                const double rr = A.rad[b0 + je] + A.rad[b0 + ke];
+                                if (dd <= cd) {
+                                    if (rce && A.max_ce > 0) {
+                                        const long long w = 4LL * A.max_ce, o = 4 * nce;
+                                        rce[o % w] = t; rce[(o + 1) % w] = jo; rce[(o + 2) % w] = ko;
+                                        rce[(o + 3) % w] = dd;
+                                    }
+                                    nce += 1;
+                                }
+                                if (rr > 0 && dd <= rr) {
+                                    if (rco && A.max_coll > 0) {
+                                        const long long w = 4LL * A.max_coll, o = 4 * nco;
+                                        rco[o % w] = t; rco[(o + 1) % w] = jo; rco[(o + 2) % w] = ko;
+                                        rco[(o + 3) % w] = dd;
+                                    }
+                                    nco += 1;
+                                }
+                            }
+                        }
+                        ctl->n_ce = nce;
+                        ctl->n_coll = nco;
+                        int st = ST_RUNNING;
+                        if (A.max_ce > 0 && nce >= A.max_ce) st = ST_CE;
+                        else if (A.max_coll > 0 && nco >= A.max_coll) st = ST_COLL;
+                        flag[1] = (double)st;
+                    }
+                    __syncthreads();
+                    const int st = (int)flag[1];
+                    if (st != ST_RUNNING) {
+                        status = st;
+                        h = h_next;
+                        break;
+                    }
+                }
+            }
+            if (finished) {
+                status = ST_DONE;
+                break;
+            }
+        }
+        h = h_next;
+        if (h < kHmin) {
+            if (!warned && tid == 0)
+                printf("Warning! Timestep %g being too small! Imposing a minimum timestep of %g, t = %g\n", h,
+                       kHmin, t);
+            warned = 1;
+            h = kHmin;
+        }
+    }
+
+    // ---- store state (y0 / dy as the reference writes pos / vel).  The
+    // pointers are laundered so that the load addresses are not kept live
+    // across the whole kernel for reuse here.
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        int b = k * NT + tid;
+        asm volatile("" : "+v"(b));
+        if (b >= n) continue;
+        const long long gi = b0 + b;
+        double *px = A.x, *py = A.y, *pz = A.z, *pvx = A.vx, *pvy = A.vy, *pvz = A.vz, *ps = A.state;
+        asm volatile("" : "+s"(px), "+s"(py), "+s"(pz), "+s"(pvx), "+s"(pvy), "+s"(pvz), "+s"(ps));
+        px[gi] = y0[k][0]; py[gi] = y0[k][1]; pz[gi] = y0[k][2];
+        pvx[gi] = dy0[k][0]; pvy[gi] = dy0[k][1]; pvz[gi] = dy0[k][2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                ps[(long long)(c * 7 + j) * T + gi] = bb[k][c][j];
+                ps[(long long)(21 + c * 7 + j) * T + gi] = gg[k][c][j];
+            }
+            ps[(long long)(42 + c) * T + gi] = a0[k][c];
+            ps[(long long)(45 + c) * T + gi] = b6p[k][c];
+        }
+    }
+    if (tid == 0) {
+        ctl->t = t;
+        ctl->h = h;
+        ctl->rejects = rejects;
+        ctl->warned = warned;
+        ctl->phase = 1;
+        ctl->status = status;
+        ctl->pad = prop_pending;
+        ctl->n_evals += n_evals;
+        ctl->n_attempts += n_att;
+        ctl->n_accepted += n_acc;
+        ctl->n_pc += n_pc;
+        ctl->n_prop_used += n_pu;
+        ctl->n_prop_rej += n_pr;
+    }
+}
+
+
+template <int NT, int BPT, int FM>
+static hipError_t launch_ct(int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    switch (ctrl) {
+        case ABIE_CTRL_PLAIN:
+            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_PLAIN>), dim3(nb), dim3(NT), lds, st, a);
+            break;
+        case ABIE_CTRL_MASSIVE:
+            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_MASSIVE>), dim3(nb), dim3(NT), lds, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_LEARNED>), dim3(nb), dim3(NT), lds, st, a);
+            break;
+    }
+    return hipGetLastError();
+}
+
+template <int FM>
+hipError_t launch_gr15_fm(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    switch (cfg) {
+        case 0: return launch_ct<64, 1, FM>(ctrl, a, nb, lds, st);
+        case 1: return launch_ct<128, 1, FM>(ctrl, a, nb, lds, st);
+        case 2: return launch_ct<256, 1, FM>(ctrl, a, nb, lds, st);
+        case 3: return launch_ct<512, 1, FM>(ctrl, a, nb, lds, st);
+        case 4: return launch_ct<512, 2, FM>(ctrl, a, nb, lds, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace abie

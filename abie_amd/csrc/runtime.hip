@@ -1,0 +1,813 @@
+// runtime.hip -- host side of libabie_hip.so: device memory, the
+// reference-compatible C ABI (src/common.c:200-413 semantics) and the
+// ensemble / force entry points of abie_hip.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "abie_hip.h"
+#include "force.hpp"
+#include "gr15.hpp"
+
+namespace abie {
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_errbuf[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_errbuf, sizeof g_errbuf, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "libabie_hip: %s\n", g_errbuf);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) return fail(ABIE_HIP_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------- device memory
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+static int device_cu_count(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount;
+}
+
+static int force_mode_from_env() {
+    const char *s = getenv("ABIE_HIP_FORCE_MODE");
+    if (!s) return ABIE_FORCE_FP64;
+    if (!strcmp(s, "exact")) return ABIE_FORCE_FP64_EXACT;
+    if (!strcmp(s, "mixed")) return ABIE_FORCE_MIXED;
+    return ABIE_FORCE_FP64;
+}
+
+// Engine order of n bodies: massive (m > 0) ascending, then the rest.
+static int engine_order(const double *m, int n, std::vector<int> &perm) {
+    perm.resize(n);
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+        if (m[i] > 0.0) perm[k++] = i;
+    const int nm = k;
+    for (int i = 0; i < n; ++i)
+        if (!(m[i] > 0.0)) perm[k++] = i;
+    return nm;
+}
+
+}  // namespace abie
+
+using namespace abie;
+
+// ===================================================================== ensemble
+struct abie_ensemble {
+    int dev = 0;
+    int n_sys = 0;
+    long long total = 0;
+    double G = 0;
+    int force_mode = ABIE_FORCE_FP64, ctrl = ABIE_CTRL_PLAIN;
+    int events = 1;
+    double ce_dist = 0.0;
+    int max_ce = 0, max_coll = 0;
+    int max_attempts = 256;
+    std::vector<int> offsets;          // [n_sys+1] original layout
+    std::vector<int> eng2orig;         // engine global index -> original global index
+    std::vector<SysDesc> desc;
+    std::vector<int> order;            // grouped by cfg
+    int group_off[kNumCfg + 1] = {0};
+    hipStream_t streams[kNumCfg] = {};
+    DBuf<SysDesc> d_sys;
+    DBuf<int> d_order, d_orig, d_inv;
+    DBuf<SysCtl> d_ctl;
+    DBuf<double> d_x, d_y, d_z, d_vx, d_vy, d_vz, d_gm, d_m, d_rad, d_ext, d_state, d_scan, d_ring_ce,
+        d_ring_coll, d_log, d_energy;
+    DBuf<unsigned char> d_del;
+    DBuf<unsigned long long> d_log_len;
+    DBuf<float> d_mlp;
+    long long log_cap = 0;
+    bool has_ext = false, has_mlp = false;
+};
+
+static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, const double *vel, const double *m,
+                     const double *r, const double *ext) {
+    const int S = e->n_sys;
+    e->offsets.assign(offsets, offsets + S + 1);
+    e->total = offsets[S];
+    const long long T = e->total;
+    e->desc.resize(S);
+    e->eng2orig.resize(T);
+    std::vector<double> x(T), y(T), z(T), vx(T), vy(T), vz(T), gm(T), mm(T), rad(T), ex;
+    std::vector<int> orig(T), inv(T);
+    std::vector<unsigned char> del(T);
+    if (ext) ex.resize(3 * T);
+    std::vector<int> perm;
+    for (int s = 0; s < S; ++s) {
+        const int o = offsets[s], n = offsets[s + 1] - offsets[s];
+        if (n <= 0) return fail(ABIE_HIP_EINVAL, "system %d has no bodies", s);
+        const int nm = engine_order(m + o, n, perm);
+        const int cfg = pick_cfg(n, nm);
+        if (cfg < 0)
+            return fail(ABIE_HIP_ETOOBIG, "system %d: %d bodies / %d massive exceeds the engine (max %d bodies, %d massive)",
+                        s, n, nm, kCfgs[kNumCfg - 1].nt * kCfgs[kNumCfg - 1].bpt, kCfgs[kNumCfg - 1].nt);
+        e->desc[s] = SysDesc{o, n, nm, cfg};
+        for (int p = 0; p < n; ++p) {
+            const int i = perm[p];
+            const long long g = o + p, gi = o + i;
+            e->eng2orig[g] = (int)gi;
+            x[g] = pos[3 * gi]; y[g] = pos[3 * gi + 1]; z[g] = pos[3 * gi + 2];
+            vx[g] = vel[3 * gi]; vy[g] = vel[3 * gi + 1]; vz[g] = vel[3 * gi + 2];
+            mm[g] = m[gi];
+            gm[g] = (p < nm) ? e->G * m[gi] : 0.0;
+            rad[g] = r ? r[gi] : 0.0;
+            del[g] = m[gi] < 0.0;
+            orig[g] = i;
+            inv[o + i] = p;
+            if (ext)
+                for (int c = 0; c < 3; ++c) ex[c * T + g] = ext[3 * gi + c];
+        }
+    }
+    // LPT order inside each configuration group (largest work first)
+    e->order.clear();
+    for (int c = 0; c < kNumCfg; ++c) {
+        e->group_off[c] = (int)e->order.size();
+        std::vector<int> grp;
+        for (int s = 0; s < S; ++s)
+            if (e->desc[s].cfg == c) grp.push_back(s);
+        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) {
+            const double wa = (double)e->desc[a].n * e->desc[a].nm, wb = (double)e->desc[b].n * e->desc[b].nm;
+            return wa > wb;
+        });
+        e->order.insert(e->order.end(), grp.begin(), grp.end());
+    }
+    e->group_off[kNumCfg] = (int)e->order.size();
+
+    HIP_TRY(e->d_sys.reserve(S));
+    HIP_TRY(e->d_order.reserve(S));
+    HIP_TRY(e->d_ctl.reserve(S));
+    HIP_TRY(e->d_energy.reserve(S));
+    for (auto *b : {&e->d_x, &e->d_y, &e->d_z, &e->d_vx, &e->d_vy, &e->d_vz, &e->d_gm, &e->d_m, &e->d_rad})
+        HIP_TRY(b->reserve(T));
+    HIP_TRY(e->d_orig.reserve(T));
+    HIP_TRY(e->d_inv.reserve(T));
+    HIP_TRY(e->d_del.reserve(T));
+    HIP_TRY(e->d_state.reserve((size_t)kStateDoubles * T));
+    HIP_TRY(e->d_scan.reserve(3 * T));
+    HIP_TRY(e->d_log_len.reserve(1));
+    e->has_ext = ext != nullptr;
+    if (ext) {
+        HIP_TRY(e->d_ext.reserve(3 * T));
+        HIP_TRY(hipMemcpy(e->d_ext.p, ex.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice));
+    }
+    auto up = [&](double *d, const std::vector<double> &h) {
+        return hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    };
+    HIP_TRY(up(e->d_x.p, x)); HIP_TRY(up(e->d_y.p, y)); HIP_TRY(up(e->d_z.p, z));
+    HIP_TRY(up(e->d_vx.p, vx)); HIP_TRY(up(e->d_vy.p, vy)); HIP_TRY(up(e->d_vz.p, vz));
+    HIP_TRY(up(e->d_gm.p, gm)); HIP_TRY(up(e->d_m.p, mm)); HIP_TRY(up(e->d_rad.p, rad));
+    HIP_TRY(hipMemcpy(e->d_orig.p, orig.data(), T * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_inv.p, inv.data(), T * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_del.p, del.data(), T, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_sys.p, e->desc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_order.p, e->order.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(e->d_state.p, 0, (size_t)kStateDoubles * T * sizeof(double)));
+    std::vector<SysCtl> ctl(S);
+    for (auto &c : ctl) {
+        memset(&c, 0, sizeof c);
+        c.status = ST_RUNNING;
+    }
+    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), S * sizeof(SysCtl), hipMemcpyHostToDevice));
+    for (int c = 0; c < kNumCfg; ++c)
+        if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
+    return 0;
+}
+
+static int ens_rings(abie_ensemble *e, int max_ce, int max_coll) {
+    e->max_ce = max_ce;
+    e->max_coll = max_coll;
+    if (max_ce > 0) {
+        HIP_TRY(e->d_ring_ce.reserve((size_t)4 * max_ce * e->n_sys));
+        HIP_TRY(hipMemset(e->d_ring_ce.p, 0, (size_t)4 * max_ce * e->n_sys * sizeof(double)));
+    }
+    if (max_coll > 0) {
+        HIP_TRY(e->d_ring_coll.reserve((size_t)4 * max_coll * e->n_sys));
+        HIP_TRY(hipMemset(e->d_ring_coll.p, 0, (size_t)4 * max_coll * e->n_sys * sizeof(double)));
+    }
+    return 0;
+}
+
+static GR15Args ens_args(abie_ensemble *e) {
+    GR15Args a;
+    memset(&a, 0, sizeof a);
+    a.sys = e->d_sys.p;
+    a.order = e->d_order.p;
+    a.ctl = e->d_ctl.p;
+    a.x = e->d_x.p; a.y = e->d_y.p; a.z = e->d_z.p;
+    a.vx = e->d_vx.p; a.vy = e->d_vy.p; a.vz = e->d_vz.p;
+    a.gm = e->d_gm.p;
+    a.rad = e->d_rad.p;
+    a.ext = e->has_ext ? e->d_ext.p : nullptr;
+    a.orig = e->d_orig.p;
+    a.inv = e->d_inv.p;
+    a.deleted = e->d_del.p;
+    a.state = e->d_state.p;
+    a.scan = e->d_scan.p;
+    a.total = e->total;
+    a.ce_dist = e->ce_dist;
+    a.max_ce = e->max_ce;
+    a.max_coll = e->max_coll;
+    a.ring_ce = e->max_ce > 0 ? e->d_ring_ce.p : nullptr;
+    a.ring_coll = e->max_coll > 0 ? e->d_ring_coll.p : nullptr;
+    a.log = e->log_cap > 0 ? e->d_log.p : nullptr;
+    a.log_len = e->d_log_len.p;
+    a.log_cap = e->log_cap;
+    a.mlp = e->has_mlp ? e->d_mlp.p : nullptr;
+    a.max_attempts = e->max_attempts;
+    a.events = e->events;
+    return a;
+}
+
+static int ens_set_t_end(abie_ensemble *e, double t_end, const double *t_start) {
+    std::vector<SysCtl> ctl(e->n_sys);
+    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->n_sys; ++s) {
+        ctl[s].t_end = t_end;
+        if (t_start) ctl[s].t = t_start[s];
+        if (ctl[s].status == ST_DONE && ctl[s].t <= t_end) ctl[s].status = ST_RUNNING;
+    }
+    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), e->n_sys * sizeof(SysCtl), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Launch every configuration group until all systems stop or time is up.
+static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<SysCtl> ctl(e->n_sys);
+    int done = 0;
+    HIP_TRY(hipSetDevice(e->dev));
+    for (;;) {
+        GR15Args a = ens_args(e);
+        for (int c = 0; c < kNumCfg; ++c) {
+            const int nb = e->group_off[c + 1] - e->group_off[c];
+            if (nb == 0) continue;
+            GR15Args ac = a;
+            ac.order = e->d_order.p + e->group_off[c];
+            HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
+        }
+        for (int c = 0; c < kNumCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
+        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        bool any_running = false;
+        for (auto &c : ctl) any_running |= (c.status == ST_RUNNING);
+        if (!any_running) {
+            done = 1;
+            break;
+        }
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (max_seconds > 0 && el >= max_seconds) break;
+    }
+    if (done_out) *done_out = done;
+    return 0;
+}
+
+extern "C" {
+
+const char *abie_hip_last_error(void) { return g_errbuf; }
+
+int abie_hip_set_device(int device) {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return fail(ABIE_HIP_ENODEV, "no HIP device");
+    if (device < 0 || device >= cnt) return fail(ABIE_HIP_ENODEV, "device %d out of range (%d devices)", device, cnt);
+    HIP_TRY(hipSetDevice(device));
+    return 0;
+}
+
+int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, const double *pos, const double *vel,
+                         const double *m, const double *r, double G, int force_mode, int controller) {
+    if (!out || n_sys <= 0 || !offsets || !pos || !vel || !m) return fail(ABIE_HIP_EINVAL, "ensemble_create: bad arguments");
+    if (force_mode < 0 || force_mode > 2 || controller < 0 || controller > 2)
+        return fail(ABIE_HIP_EINVAL, "ensemble_create: bad mode %d / controller %d", force_mode, controller);
+    auto *e = new abie_ensemble();
+    (void)hipGetDevice(&e->dev);
+    e->n_sys = n_sys;
+    e->G = G;
+    e->force_mode = force_mode;
+    e->ctrl = controller;
+    int rc = ens_build(e, offsets, pos, vel, m, r, nullptr);
+    if (rc == 0) rc = ens_rings(e, 0, 0);
+    if (rc != 0) {
+        abie_ensemble_destroy(e);
+        return rc;
+    }
+    if (const char *s = getenv("ABIE_HIP_MAX_ATTEMPTS")) e->max_attempts = std::max(1, atoi(s));
+    if (const char *s = getenv("ABIE_HIP_EVENTS")) e->events = atoi(s);
+    *out = e;
+    return 0;
+}
+
+void abie_ensemble_destroy(abie_ensemble *e) {
+    if (!e) return;
+    for (auto *b : {&e->d_x, &e->d_y, &e->d_z, &e->d_vx, &e->d_vy, &e->d_vz, &e->d_gm, &e->d_m, &e->d_rad,
+                    &e->d_ext, &e->d_state, &e->d_scan, &e->d_ring_ce, &e->d_ring_coll, &e->d_log, &e->d_energy})
+        b->release();
+    e->d_sys.release(); e->d_order.release(); e->d_orig.release(); e->d_inv.release();
+    e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release();
+    for (auto &s : e->streams)
+        if (s) (void)hipStreamDestroy(s);
+    delete e;
+}
+
+int abie_ensemble_run(abie_ensemble *e, double t_end, double max_seconds, int *done) {
+    if (!e) return fail(ABIE_HIP_EINVAL, "ensemble_run: null handle");
+    int rc = ens_set_t_end(e, t_end, nullptr);
+    if (rc) return rc;
+    return ens_run(e, max_seconds, done);
+}
+
+int abie_ensemble_get_state(abie_ensemble *e, double *pos, double *vel, double *t, int *status) {
+    if (!e) return fail(ABIE_HIP_EINVAL, "ensemble_get_state: null handle");
+    const long long T = e->total;
+    if (pos || vel) {
+        std::vector<double> a(T), b(T), c(T);
+        auto put = [&](double *dst, DBuf<double> &x, DBuf<double> &y, DBuf<double> &z) -> int {
+            HIP_TRY(hipMemcpy(a.data(), x.p, T * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(b.data(), y.p, T * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(c.data(), z.p, T * sizeof(double), hipMemcpyDeviceToHost));
+            for (long long g = 0; g < T; ++g) {
+                const long long i = e->eng2orig[g];
+                dst[3 * i] = a[g]; dst[3 * i + 1] = b[g]; dst[3 * i + 2] = c[g];
+            }
+            return 0;
+        };
+        if (pos && put(pos, e->d_x, e->d_y, e->d_z)) return ABIE_HIP_EHIP;
+        if (vel && put(vel, e->d_vx, e->d_vy, e->d_vz)) return ABIE_HIP_EHIP;
+    }
+    if (t || status) {
+        std::vector<SysCtl> ctl(e->n_sys);
+        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        for (int s = 0; s < e->n_sys; ++s) {
+            if (t) t[s] = ctl[s].t;
+            if (status) status[s] = ctl[s].status == ST_DONE ? 0 : ctl[s].status;
+        }
+    }
+    return 0;
+}
+
+int abie_ensemble_get_counters(abie_ensemble *e, long long *cnt) {
+    if (!e || !cnt) return fail(ABIE_HIP_EINVAL, "ensemble_get_counters: bad arguments");
+    std::vector<SysCtl> ctl(e->n_sys);
+    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->n_sys; ++s) {
+        long long *c = cnt + 6 * s;
+        c[0] = ctl[s].n_evals; c[1] = ctl[s].n_attempts; c[2] = ctl[s].n_accepted;
+        c[3] = ctl[s].n_pc; c[4] = ctl[s].n_prop_used; c[5] = ctl[s].n_prop_rej;
+    }
+    return 0;
+}
+
+int abie_ensemble_energy(abie_ensemble *e, double *energy) {
+    if (!e || !energy) return fail(ABIE_HIP_EINVAL, "ensemble_energy: bad arguments");
+    HIP_TRY(launch_energy(e->d_sys.p, e->n_sys, e->d_x.p, e->d_y.p, e->d_z.p, e->d_vx.p, e->d_vy.p, e->d_vz.p,
+                          e->d_m.p, e->G, e->d_energy.p, 0));
+    HIP_TRY(hipMemcpy(energy, e->d_energy.p, e->n_sys * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int abie_ensemble_set_steplog(abie_ensemble *e, long long capacity) {
+    if (!e || capacity < 0) return fail(ABIE_HIP_EINVAL, "ensemble_set_steplog: bad arguments");
+    e->log_cap = capacity;
+    if (capacity > 0) HIP_TRY(e->d_log.reserve((size_t)capacity * ABIE_STEPLOG_FIELDS));
+    HIP_TRY(hipMemset(e->d_log_len.p, 0, sizeof(unsigned long long)));
+    return 0;
+}
+
+int abie_ensemble_get_steplog(abie_ensemble *e, double *records, long long *n_records) {
+    if (!e || !n_records) return fail(ABIE_HIP_EINVAL, "ensemble_get_steplog: bad arguments");
+    unsigned long long len = 0;
+    HIP_TRY(hipMemcpy(&len, e->d_log_len.p, sizeof len, hipMemcpyDeviceToHost));
+    const long long n = std::min<long long>((long long)len, e->log_cap);
+    if (records && n > 0)
+        HIP_TRY(hipMemcpy(records, e->d_log.p, (size_t)n * ABIE_STEPLOG_FIELDS * sizeof(double), hipMemcpyDeviceToHost));
+    *n_records = n;
+    return 0;
+}
+
+int abie_ensemble_set_proposer(abie_ensemble *e, const float *w, int n_weights) {
+    if (!e) return fail(ABIE_HIP_EINVAL, "ensemble_set_proposer: null handle");
+    if (!w) {
+        e->has_mlp = false;
+        return 0;
+    }
+    if (n_weights != ABIE_MLP_NPARAM) return fail(ABIE_HIP_EINVAL, "proposer needs %d weights, got %d", ABIE_MLP_NPARAM, n_weights);
+    HIP_TRY(e->d_mlp.reserve(ABIE_MLP_NPARAM));
+    HIP_TRY(hipMemcpy(e->d_mlp.p, w, ABIE_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    e->has_mlp = true;
+    return 0;
+}
+
+}  // extern "C"
+
+// ===================================================================== reference ABI
+namespace {
+
+struct GlobalSim {
+    bool inited = false;
+    int dev = -1;
+    double G = 0, C = 0;
+    size_t nmax = 0, max_ce = 0, max_coll = 0;
+    size_t n = 0;
+    std::vector<double> pos, vel, m, r, ext;
+    bool ext_on = false;
+    double t = 0.0, ce_dist = 0.0;
+    size_t n_ce = 0, n_coll = 0;
+    std::vector<double> buf_ce, buf_coll;
+    int force_mode = ABIE_FORCE_FP64;
+    bool mode_set = false;
+    // AoS force path buffers
+    DBuf<double> d_pos, d_m, d_x, d_y, d_z, d_gm, d_ax, d_ay, d_az, d_acc, d_ext, d_part;
+    DBuf<int> d_perm;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+};
+
+GlobalSim g;
+
+int check_device() {
+    if (!g.inited) return fail(ABIE_HIP_ESTATE, "initialize_code has not been called");
+    HIP_TRY(hipSetDevice(g.dev));
+    return 0;
+}
+
+// Host AoS accelerations through the device (common.c:82-114 semantics).
+int aos_accel(const double *pos, size_t N, double G, const double *m, double *acc, bool with_ext) {
+    if (N == 0) return 0;
+    int rc = check_device();
+    if (rc) return rc;
+    std::vector<int> perm;
+    const int nm = engine_order(m, (int)N, perm);
+    bool any_deleted = false;
+    for (size_t i = 0; i < N; ++i) any_deleted |= (m[i] < 0.0);
+    HIP_TRY(g.d_pos.reserve(3 * N)); HIP_TRY(g.d_m.reserve(N)); HIP_TRY(g.d_perm.reserve(N));
+    for (auto *b : {&g.d_x, &g.d_y, &g.d_z, &g.d_gm, &g.d_ax, &g.d_ay, &g.d_az}) HIP_TRY(b->reserve(N));
+    HIP_TRY(g.d_acc.reserve(3 * N));
+    hipStream_t st = g.stream;
+    HIP_TRY(hipMemcpyAsync(g.d_pos.p, pos, 3 * N * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(g.d_m.p, m, N * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(g.d_perm.p, perm.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+    if (any_deleted) HIP_TRY(hipMemcpyAsync(g.d_acc.p, acc, 3 * N * sizeof(double), hipMemcpyHostToDevice, st));
+    const double *dext = nullptr;
+    if (with_ext && g.ext_on && g.ext.size() >= 3 * N) {
+        HIP_TRY(g.d_ext.reserve(3 * N));
+        HIP_TRY(hipMemcpyAsync(g.d_ext.p, g.ext.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, st));
+        dext = g.d_ext.p;
+    }
+    HIP_TRY(launch_pack(g.d_pos.p, g.d_m.p, g.d_perm.p, (int)N, nm, G, g.d_x.p, g.d_y.p, g.d_z.p, g.d_gm.p, st));
+    AccelArgs a{g.d_x.p, g.d_y.p, g.d_z.p, g.d_gm.p, g.d_x.p, g.d_y.p, g.d_z.p, g.d_ax.p, g.d_ay.p, g.d_az.p,
+                nm, (int)N, nm, g.force_mode};
+    AccelPlan pl = plan_accel(nm, (int)N, g.force_mode, g.n_cu);
+    if (pl.parts > 1) HIP_TRY(g.d_part.reserve((size_t)pl.parts * 3 * N));
+    HIP_TRY(launch_accel(a, pl, g.d_part.p, st));
+    HIP_TRY(launch_unpack(g.d_perm.p, g.d_m.p, (int)N, g.d_ax.p, g.d_ay.p, g.d_az.p, dext, g.d_acc.p, st));
+    HIP_TRY(hipMemcpyAsync(acc, g.d_acc.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+// GR15 of the global simulation through a one-system ensemble.
+int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t N, double G, double t,
+             double t_end, double *t_out) {
+    if (N == 0) return fail(ABIE_HIP_EINVAL, "integrator_gr: no bodies (the reference never terminates here)");
+    if (g.C > 0.0) return fail(ABIE_HIP_EUNSUP, "post-Newtonian forces (C > 0) are not part of the device path");
+    int rc = check_device();
+    if (rc) return rc;
+    abie_ensemble *e = new abie_ensemble();
+    e->dev = g.dev;
+    e->n_sys = 1;
+    e->G = G;
+    e->force_mode = g.force_mode;
+    e->ctrl = ABIE_CTRL_PLAIN;
+    e->ce_dist = g.ce_dist;
+    e->max_attempts = 4096;
+    if (const char *s = getenv("ABIE_HIP_MAX_ATTEMPTS")) e->max_attempts = std::max(1, atoi(s));
+    const int offs[2] = {0, (int)N};
+    const double *ext = (g.ext_on && g.ext.size() >= 3 * N) ? g.ext.data() : nullptr;
+    rc = ens_build(e, offs, pos, vel, m, r, ext);
+    if (rc == 0) rc = ens_rings(e, (int)g.max_ce, (int)g.max_coll);
+    // carry the event counters and rings of the global simulation
+    if (rc == 0) {
+        SysCtl c;
+        memset(&c, 0, sizeof c);
+        c.status = ST_RUNNING;
+        c.t = t;
+        c.t_end = t_end;
+        c.n_ce = (long long)g.n_ce;
+        c.n_coll = (long long)g.n_coll;
+        if (hipMemcpy(e->d_ctl.p, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess) rc = ABIE_HIP_EHIP;
+        if (rc == 0 && g.max_ce > 0)
+            if (hipMemcpy(e->d_ring_ce.p, g.buf_ce.data(), 4 * g.max_ce * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                rc = ABIE_HIP_EHIP;
+        if (rc == 0 && g.max_coll > 0)
+            if (hipMemcpy(e->d_ring_coll.p, g.buf_coll.data(), 4 * g.max_coll * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                rc = ABIE_HIP_EHIP;
+    }
+    int done = 0;
+    if (rc == 0) rc = ens_run(e, 0.0, &done);
+    int ret = 0;
+    if (rc == 0) {
+        SysCtl c;
+        if (hipMemcpy(&c, e->d_ctl.p, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) rc = ABIE_HIP_EHIP;
+        if (rc == 0) rc = abie_ensemble_get_state(e, pos, vel, nullptr, nullptr);
+        if (rc == 0) {
+            *t_out = c.t;
+            g.n_ce = (size_t)c.n_ce;
+            g.n_coll = (size_t)c.n_coll;
+            if (g.max_ce > 0)
+                (void)hipMemcpy(g.buf_ce.data(), e->d_ring_ce.p, 4 * g.max_ce * sizeof(double), hipMemcpyDeviceToHost);
+            if (g.max_coll > 0)
+                (void)hipMemcpy(g.buf_coll.data(), e->d_ring_coll.p, 4 * g.max_coll * sizeof(double), hipMemcpyDeviceToHost);
+            ret = c.status == ST_CE ? ABIE_EXIT_MAX_N_CE_EXCEEDED
+                  : c.status == ST_COLL ? ABIE_EXIT_MAX_N_COLLISIONS_EXCEEDED : ABIE_EXIT_NORMAL;
+            if (c.status < 0) rc = fail(ABIE_HIP_EHIP, "integrator_gr: device engine reported an error");
+        }
+    }
+    abie_ensemble_destroy(e);
+    return rc ? rc : ret;
+}
+
+}  // namespace
+
+extern "C" {
+
+int initialize_code(double G, double C, size_t N_MAX, size_t MAX_N_CE, size_t MAX_N_COLLISIONS, int deviceID) {
+    if (g.inited) return 0;  // once per process (common.c:225-227)
+    printf("Initializing the code..., deviceID = %d", deviceID);
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
+        printf("\n");
+        return fail(ABIE_HIP_ENODEV, "no HIP device found; libabie_hip.so has no CPU path");
+    }
+    if (deviceID < 0 || deviceID >= cnt) {
+        printf("\n");
+        return fail(ABIE_HIP_ENODEV, "deviceID %d invalid (%d devices); use the reference libabie.so for the CPU path",
+                    deviceID, cnt);
+    }
+    HIP_TRY(hipSetDevice(deviceID));
+    g.dev = deviceID;
+    g.n_cu = device_cu_count(deviceID);
+    if (!g.stream) HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    g.G = G;
+    g.C = C;
+    g.nmax = N_MAX;
+    g.max_ce = MAX_N_CE;
+    g.max_coll = MAX_N_COLLISIONS;
+    g.pos.assign(3 * N_MAX, 0.0);
+    g.vel.assign(3 * N_MAX, 0.0);
+    g.m.assign(N_MAX, 0.0);
+    g.r.assign(N_MAX, 0.0);
+    if (g.ext.size() < 3 * N_MAX) g.ext.assign(3 * N_MAX, 0.0);
+    g.buf_ce.assign(4 * MAX_N_CE, 0.0);
+    g.buf_coll.assign(4 * MAX_N_COLLISIONS, 0.0);
+    g.n_ce = 0;
+    if (!g.mode_set) g.force_mode = force_mode_from_env();
+    g.inited = true;
+    printf("Initialized.\n");
+    return 0;
+}
+
+int finalize_code(void) {
+    for (auto *b : {&g.d_pos, &g.d_m, &g.d_x, &g.d_y, &g.d_z, &g.d_gm, &g.d_ax, &g.d_ay, &g.d_az, &g.d_acc, &g.d_ext, &g.d_part})
+        b->release();
+    g.d_perm.release();
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    g.stream = nullptr;
+    g.inited = false;
+    g.n = 0;
+    return 0;
+}
+
+void set_state(double *pos, double *vel, double *m, double *r, size_t N, double G, double C) {
+    if (N > g.nmax) {
+        g.nmax = N;
+        g.pos.resize(3 * N); g.vel.resize(3 * N); g.m.resize(N); g.r.resize(N);
+        if (g.ext.size() < 3 * N) g.ext.resize(3 * N, 0.0);
+    }
+    std::copy(pos, pos + 3 * N, g.pos.begin());
+    std::copy(vel, vel + 3 * N, g.vel.begin());
+    std::copy(m, m + N, g.m.begin());
+    std::copy(r, r + N, g.r.begin());
+    g.n = N;
+    g.G = G;
+    g.C = C;
+}
+
+int get_state(double *pos, double *vel, double *m, double *r) {
+    std::copy(g.pos.begin(), g.pos.begin() + 3 * g.n, pos);
+    std::copy(g.vel.begin(), g.vel.begin() + 3 * g.n, vel);
+    std::copy(g.m.begin(), g.m.begin() + g.n, m);
+    std::copy(g.r.begin(), g.r.begin() + g.n, r);
+    return (int)g.n;
+}
+
+double get_model_time(void) { return g.t; }
+void set_close_encounter_distance(double d) { g.ce_dist = d; }
+double get_close_encounter_distance(void) { return g.ce_dist; }
+void get_close_encounter_buffer(double *buf) { std::copy(g.buf_ce.begin(), g.buf_ce.end(), buf); }
+void get_collision_buffer(double *buf) { std::copy(g.buf_coll.begin(), g.buf_coll.end(), buf); }
+void reset_close_encounter_buffer(void) {
+    g.n_ce = 0;
+    std::fill(g.buf_ce.begin(), g.buf_ce.end(), 0.0);
+}
+void reset_collision_buffer(void) {
+    g.n_coll = 0;
+    std::fill(g.buf_coll.begin(), g.buf_coll.end(), 0.0);
+}
+
+size_t set_additional_forces(size_t N, double *ext_acc) {
+    g.ext_on = true;
+    if (g.ext.size() < 3 * N) g.ext.resize(3 * N, 0.0);
+    std::copy(ext_acc, ext_acc + 3 * N, g.ext.begin());
+    return 0;
+}
+
+double calculate_energy(void) {
+    if (check_device() != 0 || g.n == 0) return 0.0;
+    const size_t N = g.n;
+    DBuf<SysDesc> sd;
+    DBuf<double> buf, out;
+    double e = 0.0;
+    if (sd.reserve(1) != hipSuccess || buf.reserve(7 * N) != hipSuccess || out.reserve(1) != hipSuccess) {
+        fail(ABIE_HIP_ENOMEM, "calculate_energy: allocation failed");
+        return 0.0;
+    }
+    std::vector<double> h(7 * N);
+    for (size_t i = 0; i < N; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            h[c * N + i] = g.pos[3 * i + c];
+            h[(3 + c) * N + i] = g.vel[3 * i + c];
+        }
+        h[6 * N + i] = g.m[i];
+    }
+    SysDesc d{0, (int)N, 0, 0};
+    bool ok = hipMemcpy(sd.p, &d, sizeof d, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(buf.p, h.data(), 7 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              launch_energy(sd.p, 1, buf.p, buf.p + N, buf.p + 2 * N, buf.p + 3 * N, buf.p + 4 * N, buf.p + 5 * N,
+                            buf.p + 6 * N, g.G, out.p, 0) == hipSuccess &&
+              hipMemcpy(&e, out.p, sizeof e, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) fail(ABIE_HIP_EHIP, "calculate_energy: device call failed");
+    sd.release(); buf.release(); out.release();
+    return e;
+}
+
+int integrator_gr(double t, double t_end, double dt) {
+    (void)dt;
+    if (!g.inited) return fail(ABIE_HIP_ESTATE, "initialize_code has not been called");
+    double t_out = t;
+    const int ret = run_gr15(g.pos.data(), g.vel.data(), g.m.data(), g.r.data(), g.n, g.G, t, t_end, &t_out);
+    if (ret >= 0) g.t = t_out;
+    return ret;
+}
+
+size_t integrator_gauss_radau15(double *pos, double *vel, double *m, double *r, size_t N, double G, double t,
+                                double t_end, double dt) {
+    (void)dt;
+    double t_out = t;
+    const int ret = run_gr15(pos, vel, m, r, N, G, t, t_end, &t_out);
+    if (ret >= 0) g.t = t_out;
+    return (size_t)(long long)ret;
+}
+
+size_t calculate_accelerations(const double *pos, const double *vel, size_t N, double G, const double *m,
+                               const double *r, double *acc) {
+    (void)vel;
+    (void)r;
+    if (g.C > 0.0) return (size_t)(long long)fail(ABIE_HIP_EUNSUP, "post-Newtonian forces (C > 0) are not part of the device path");
+    return (size_t)(long long)aos_accel(pos, N, G, m, acc, true);
+}
+
+size_t ode_n_body_second_order(const double *pos, size_t N, double G, const double *m, const double *r, double *acc) {
+    (void)r;
+    return (size_t)(long long)aos_accel(pos, N, G, m, acc, false);
+}
+
+size_t ode_n_body_second_order_gpu(const double *pos, size_t N, double G, const double *m, const double *r,
+                                   double *acc) {
+    return ode_n_body_second_order(pos, N, G, m, r, acc);
+}
+
+size_t check_collisions_close_encounters(const double *pos, const double *radii, size_t N, double t) {
+    if (check_device() != 0) return (size_t)(long long)ABIE_HIP_ESTATE;
+    if (N == 0) return 0;
+    DBuf<double> dpos, drad, dce, dco;
+    DBuf<long long> dcnt;
+    DBuf<int> dret;
+    long long cnt[2] = {(long long)g.n_ce, (long long)g.n_coll};
+    int ret = 0;
+    bool ok = dpos.reserve(3 * N) == hipSuccess && drad.reserve(N) == hipSuccess && dcnt.reserve(2) == hipSuccess &&
+              dret.reserve(1) == hipSuccess && dce.reserve(4 * std::max<size_t>(g.max_ce, 1)) == hipSuccess &&
+              dco.reserve(4 * std::max<size_t>(g.max_coll, 1)) == hipSuccess;
+    ok = ok && hipMemcpy(dpos.p, pos, 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(drad.p, radii, N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dcnt.p, cnt, sizeof cnt, hipMemcpyHostToDevice) == hipSuccess &&
+         (g.max_ce == 0 || hipMemcpy(dce.p, g.buf_ce.data(), 4 * g.max_ce * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) &&
+         (g.max_coll == 0 || hipMemcpy(dco.p, g.buf_coll.data(), 4 * g.max_coll * sizeof(double), hipMemcpyHostToDevice) == hipSuccess);
+    ok = ok && launch_events(dpos.p, drad.p, (int)N, t, g.ce_dist, dcnt.p, (int)g.max_ce, (int)g.max_coll, dce.p, dco.p,
+                             dret.p, g.stream) == hipSuccess &&
+         hipStreamSynchronize(g.stream) == hipSuccess &&
+         hipMemcpy(cnt, dcnt.p, sizeof cnt, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(&ret, dret.p, sizeof ret, hipMemcpyDeviceToHost) == hipSuccess &&
+         (g.max_ce == 0 || hipMemcpy(g.buf_ce.data(), dce.p, 4 * g.max_ce * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) &&
+         (g.max_coll == 0 || hipMemcpy(g.buf_coll.data(), dco.p, 4 * g.max_coll * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess);
+    dpos.release(); drad.release(); dce.release(); dco.release(); dcnt.release(); dret.release();
+    if (!ok) return (size_t)(long long)fail(ABIE_HIP_EHIP, "check_collisions_close_encounters: device call failed");
+    g.n_ce = (size_t)cnt[0];
+    g.n_coll = (size_t)cnt[1];
+    return (size_t)ret;
+}
+
+int abie_hip_set_force_mode(int mode) {
+    if (mode < 0 || mode > 2) return fail(ABIE_HIP_EINVAL, "bad force mode %d", mode);
+    g.force_mode = mode;
+    g.mode_set = true;
+    return 0;
+}
+
+void gpu_init(int N, int deviceID) {
+    if (!g.inited) initialize_code(g.G, g.C, (size_t)std::max(N, 0), 1, 1, deviceID);
+}
+
+void gpu_finalize(void) { finalize_code(); }
+
+int integrator_rk(double, double, double) { return fail(ABIE_HIP_EUNSUP, "RungeKutta is outside the device path"); }
+int integrator_wh(double, double, double) { return fail(ABIE_HIP_EUNSUP, "WisdomHolman is outside the device path"); }
+size_t ode_n_body_first_order(double *, size_t, double, const double *, double *) {
+    return (size_t)(long long)fail(ABIE_HIP_EUNSUP, "ode_n_body_first_order is outside the device path");
+}
+
+// ---------------------------------------------------------------- SoA force
+int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz, const double *gm, int n_src,
+                       const double *px, const double *py, const double *pz, int n_sink, int n_self, double *ax,
+                       double *ay, double *az, int mode, void *stream) {
+    if (n_src < 0 || n_sink < 0 || n_self < 0 || n_self > n_src || n_self > n_sink)
+        return fail(ABIE_HIP_EINVAL, "accel_soa: bad sizes (n_src %d, n_sink %d, n_self %d)", n_src, n_sink, n_self);
+    if (mode < 0 || mode > 2) return fail(ABIE_HIP_EINVAL, "accel_soa: bad mode %d", mode);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    static thread_local DBuf<double> part;
+    AccelArgs a{sx, sy, sz, gm, px, py, pz, ax, ay, az, n_src, n_sink, n_self, mode};
+    AccelPlan pl = plan_accel(n_src, n_sink, mode, device_cu_count(dev));
+    if (pl.parts > 1) HIP_TRY(part.reserve((size_t)pl.parts * 3 * n_sink));
+    HIP_TRY(launch_accel(a, pl, part.p, (hipStream_t)stream));
+    return 0;
+}
+
+int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz, const double *gm, int n_src,
+                            const double *px, const double *py, const double *pz, int n_sink, int n_self,
+                            double *ax, double *ay, double *az, int mode, void *stream, int reps,
+                            double *ms_per_launch) {
+    if (reps <= 0 || !ms_per_launch) return fail(ABIE_HIP_EINVAL, "time_accel_soa: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = 0;
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) {
+        rc = abie_hip_accel_soa(sx, sy, sz, gm, n_src, px, py, pz, n_sink, n_self, ax, ay, az, mode, stream);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_launch = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+}  // extern "C"

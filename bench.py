@@ -1,0 +1,222 @@
+"""Benchmark of the MI355X all-pairs gravity path (libabie_hip.so).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
+
+A "step" is one acceleration evaluation of the force-kernel workload: 64
+massive bodies (star + 63, m log-U[1e-9, 1e-3]) plus 1,048,576 test
+particles (a ~ U[1, 100] au), FP64, inputs resident in HBM.  value =
+algorithmic interactions (N_m (N_m - 1) + N_t N_m = 67,112,896 per step) per
+second over the K timed steps.  The kernel's mean launch time comes from HIP
+events on the launch stream (roofline.achieved), its HBM traffic from the
+committed rocprofv3 counter summary (profiles/force_c1a_traffic.json).
+
+Extras (reported beside the headline, not part of it): the 65,536-body
+Plummer force case, the mixed-precision force mode, the drop-in GR15 on the
+1 star + 4 planets + 1,000 test particle system, and a sample of the 4,096
+system ensemble in plain FP64 and in the chosen form.
+
+The CPU baseline times the reference's own C force loop (oracle/_ref, built
+from the reference sources) -- or the oracle's restatement when that is
+absent -- on one host core, on a bounded sample (64 + 65,536 test particles).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+G = 4 * np.pi ** 2
+FLOP_PER_INTERACTION = 20          # 3 sub, 6 for r^2, 4 for r^-3, 1 for m r^-3, 6 accumulate
+FP64_PEAK_TFLOPS = 78.6            # MI355X FP64 vector peak
+METRIC = "interactions/s & system-yr/s at |dE/E|<=1e-10, chosen form vs plain FP64 GR15"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def interactions(n_m, n_t):
+    return n_m * (n_m - 1) + n_t * n_m
+
+
+def force_case(pos, m, Gc, mode, steps, warmup):
+    import torch
+    from abie_amd import engine
+    src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, Gc)
+    out = tuple(torch.empty(len(m), dtype=torch.float64, device="cuda") for _ in range(3))
+    for _ in range(warmup):
+        engine.accel_soa(src, sinks, n_self, mode, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_event = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ms_event, n_self
+
+
+def cpu_baseline(n_test=65536, reps=3):
+    """Reference C force loop on one host core (oracle/_ref), else the oracle."""
+    import ctypes
+    from abie_amd import ics
+    pos, vel, m = ics.config1a(n_test=n_test)
+    kind = "reference"
+    ref = os.path.join(ROOT, "oracle", "_ref", "libabie_ref.so")
+    dp = ctypes.POINTER(ctypes.c_double)
+    p = np.ascontiguousarray(pos.reshape(-1))
+    mm = np.ascontiguousarray(m)
+    acc = np.zeros_like(p)
+    r = np.zeros_like(mm)
+    try:
+        L = ctypes.CDLL(ref)
+        f = L.ode_n_body_second_order
+        f.argtypes = [dp, ctypes.c_size_t, ctypes.c_double, dp, dp, dp]
+        call = lambda: f(p.ctypes.data_as(dp), len(mm), G, mm.ctypes.data_as(dp), r.ctypes.data_as(dp),
+                         acc.ctypes.data_as(dp))
+    except OSError:
+        from oracle import oracle
+        kind = "port"
+        call = lambda: oracle.accel(pos, G, m)
+    call()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    dt = (time.perf_counter() - t0) / reps
+    n_int = interactions(64, n_test)
+    return {"value": n_int / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
+            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each" % (n_test, reps, dt)}
+
+
+def gr15_dropin(years, force_mode="fp64"):
+    """config-0 system through the reference-compatible C ABI (device=0)."""
+    from abie_amd import ics
+    from abie_amd.clibabie import CLibABIE
+    pos, vel, m = ics.config0()
+    c = CLibABIE()
+    c.lib.finalize_code()
+    c.initialize_code(G, 0.0, len(m), 1, 1, 0.0, deviceID=0)
+    c.lib.abie_hip_set_force_mode({"fp64": 0, "exact": 1, "mixed": 2}[force_mode])
+    p, v = pos.reshape(-1).copy(), vel.reshape(-1).copy()
+    c.set_state(p, v, m, np.zeros_like(m), len(m), G, 0.0)
+    e0 = c.get_total_energy()
+    t0 = time.perf_counter()
+    c.integrator_gr(0.0, years, 1.0)
+    wall = time.perf_counter() - t0
+    t = c.get_model_time()
+    c.get_state(p, v, m.copy(), np.zeros_like(m))
+    e1 = c.get_total_energy()
+    c.lib.abie_hip_set_force_mode(0)
+    c.lib.finalize_code()
+    return {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "dE_E": abs((e1 - e0) / e0)}
+
+
+def ensemble_sample(n_sys, years, force_mode, controller, proposer=None):
+    from abie_amd import engine, ics
+    systems = ics.ensemble(n_sys)
+    ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller=controller)
+    if proposer is not None:
+        ens.set_proposer(proposer)
+    e0 = ens.energy()
+    t0 = time.perf_counter()
+    done = ens.run(years, max_seconds=120.0)
+    wall = time.perf_counter() - t0
+    _, _, t, st = ens.state()
+    e1 = ens.energy()
+    cnt = ens.counters()
+    ens.close()
+    de = np.abs((e1 - e0) / e0)
+    return {"n_systems": n_sys, "model_yr_each": years, "done": bool(done), "wall_s": wall,
+            "system_yr_per_s": float(t.sum() / wall), "max_dE_E": float(de.max()),
+            "evals_per_yr": float(cnt[:, 0].sum() / t.sum()),
+            "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1))}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "force_c1a_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
+              "(the ensemble shards by system with no communication).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from abie_amd import _lib, ics
+    _lib.load()
+    torch.cuda.init()
+
+    pos, vel, m = ics.config1a()
+    n_m, n_t = int((m > 0).sum()), int((m == 0).sum())
+    n_int = interactions(n_m, n_t)
+    log("force workload: %d massive + %d test particles, %d interactions/step" % (n_m, n_t, n_int))
+    wall, ms_event, _ = force_case(pos, m, G, "fp64", args.steps, args.warmup)
+    value = n_int * args.steps / wall
+    achieved = n_int * FLOP_PER_INTERACTION / (ms_event * 1e-3) / 1e12
+    log("fp64: %.3f us/launch (events), %.4g interactions/s" % (ms_event * 1e3, value))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "force kernel, 64 massive + 1,048,576 test particles (a~U[1,100] au), "
+                               "one acceleration evaluation per step",
+                   "n_massive": n_m, "n_test": n_t, "interactions_per_step": n_int, "mode": "fp64"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
+                     "kernel": "accel_kernel<4, FP64>", "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "mean_launch_us": ms_event * 1e3},
+    }
+    extras = {}
+    if not args.no_extras:
+        reps = max(10, args.steps // 20)
+        pp, pv, pm = ics.plummer(65536)
+        w, ms, _ = force_case(pp, pm, 1.0, "fp64", reps, 3)
+        extras["plummer65536_fp64"] = {"interactions_per_s": interactions(65536, 0) * reps / w,
+                                       "mean_launch_us": ms * 1e3,
+                                       "frac_fp64_peak": interactions(65536, 0) * 20 / (ms * 1e-3) / 1e12
+                                       / FP64_PEAK_TFLOPS}
+        log("plummer: %s" % extras["plummer65536_fp64"])
+        w, ms, _ = force_case(pos, m, G, "mixed", args.steps, args.warmup)
+        extras["config1a_mixed"] = {"interactions_per_s": n_int * args.steps / w, "mean_launch_us": ms * 1e3}
+        w, ms, _ = force_case(pos, m, G, "exact", reps, 3)
+        extras["config1a_exact"] = {"interactions_per_s": n_int * reps / w, "mean_launch_us": ms * 1e3}
+        log("mixed/exact: %s %s" % (extras["config1a_mixed"], extras["config1a_exact"]))
+        extras["gr15_dropin_config0_10yr"] = gr15_dropin(10.0)
+        log("gr15 drop-in: %s" % extras["gr15_dropin_config0_10yr"])
+    out["extras"] = extras
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+        log("cpu: %s" % out["cpu_baseline"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
