@@ -1,0 +1,64 @@
+"""Drop-in facade pieces that need no GPU: particles, element conversions,
+plugin discovery, snapshot buffer."""
+import numpy as np
+import pytest
+
+G = 4 * np.pi ** 2
+
+
+def test_elements_round_trip_and_invariants():
+    from abie_amd.tools import cartesian_to_elements, elements_to_cartesian
+    rng = np.random.default_rng(0)
+    n = 200
+    a = rng.uniform(0.5, 50, n); e = rng.uniform(0.01, 0.8, n); i = rng.uniform(0.01, 3.0, n)
+    Om = rng.uniform(-3, 3, n); om = rng.uniform(-3, 3, n); f = rng.uniform(-3, 3, n)
+    mp = rng.uniform(0, 1e-3, n)
+    pos, vel = elements_to_cartesian(mp, 1.0, a, e, i, Om, om, f, G=G)
+    mu = G * (1.0 + mp)
+    r = np.linalg.norm(pos, axis=1); v2 = (vel ** 2).sum(1)
+    # vis-viva and |h|^2 = mu a (1 - e^2), periapsis distance
+    np.testing.assert_allclose(v2, mu * (2 / r - 1 / a), rtol=1e-12)
+    h2 = (np.cross(pos, vel) ** 2).sum(1)
+    np.testing.assert_allclose(h2, mu * a * (1 - e ** 2), rtol=1e-12)
+    a2, e2, i2, Om2, om2, f2 = cartesian_to_elements(mp, 1.0, pos, vel, G)
+    np.testing.assert_allclose(a2, a, rtol=1e-10)
+    np.testing.assert_allclose(e2, e, rtol=1e-9)
+    np.testing.assert_allclose(i2, i, rtol=1e-10)
+    wrap = lambda x: (x + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(wrap(Om2 - Om)).max() < 1e-9 and np.abs(wrap(om2 - om)).max() < 1e-8
+    assert np.abs(wrap(f2 - f)).max() < 1e-8
+
+
+def test_particles_add_primary_com_and_merge():
+    from abie_amd.particles import Particles
+    p = Particles(G)
+    p.add(mass=1.0, name="Sun")
+    p.add(mass=1e-3, a=1.0, e=0.0, f=0.0, name="P")
+    # default primary: COM of the bodies added before (here the Sun alone)
+    assert np.allclose(p["P"].pos, [1.0, 0, 0])
+    p.add(mass=0.0, a=2.0, f=0.0)
+    com = (1.0 * np.zeros(3) + 1e-3 * np.array([1.0, 0, 0])) / 1.001
+    assert np.allclose(p[2].pos, com + [2.0, 0, 0])
+    assert p.N == 3 and p.positions.shape == (9,)
+    p.add(mass=2e-3, pos=[1.0, 0.0, 0.0], vel=[0, 1.0, 0], radius=0.2, name="Q")
+    v_before = (1e-3 * p[1].vel + 2e-3 * p[3].vel) / 3e-3
+    p.merge_particles_inelastically(1, 3)
+    assert p.N == 3
+    assert np.allclose(p[2].vel, v_before) and p.masses[2] == pytest.approx(3e-3)
+
+
+def test_plugins_keep_reference_names():
+    from abie_amd.integrator import Integrator
+    names = set(Integrator.load_integrators())
+    assert names >= {"GaussRadau15", "WisdomHolman", "RungeKutta", "AdamsBashforth", "LeapFrog", "Euler"}
+
+
+def test_dataio_flush(tmp_path):
+    from abie_amd.data_io import DataIO
+    d = DataIO(buf_len=3, output_file_name=str(tmp_path / "sim.hdf5"))
+    for k in range(5):
+        d.store_state(float(k), np.arange(6.0), np.ones(6), np.array([1.0, 0.0]), energy=-1.0)
+    d.close()
+    z = np.load(tmp_path / "sim.npz")
+    assert len([k for k in z.files if k.endswith("/t")]) == 5
+    assert z["Step#4/t"] == 4.0
