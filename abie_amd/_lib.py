@@ -83,9 +83,17 @@ class AbieHipError(RuntimeError):
 
 
 def load():
-    """Load libabie_hip.so (raises if it has not been built)."""
+    """Load libabie_hip.so (raises if it has not been built).
+
+    When PyTorch is importable it is imported first: torch ships its own HIP
+    runtime, and loading it before this library makes both use that one
+    runtime (same soname) instead of two in one process."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise AbieHipError("libabie_hip.so not built: run `python -m abie_amd.build` (%s)" % LIB_PATH)
         lib = ctypes.CDLL(LIB_PATH)

@@ -105,7 +105,7 @@ ABIE_HD double pow_pos(double x, double y) {
     if (x < 0.0) return __builtin_nan("");
     if (x == 0.0) return 0.0;
     if (x == 1.0) return 1.0;
-    if (isinf(x)) return x;
+    if (x > DBL_MAX) return x;  // +inf
     const dd l = dd_log(x);
     const dd z = dd_mul_d(l, y);
     const dd r = dd_exp(z);

@@ -42,7 +42,7 @@ struct GR15Args {
     const double *rad;                 // radii
     const double *ext;                 // [3][total] constant extra acceleration, or null
     const int *orig;                   // original index within the system
-    const int *inv;                    // engine index of original index (per system)
+    const int *inv;                    // the unit's engine indices by ascending original index
     const unsigned char *deleted;      // m < 0
     double *state;                     // [kStateDoubles][total]
     double *scan;                      // [3][total] event-scan positions

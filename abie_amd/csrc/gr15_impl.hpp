@@ -396,7 +396,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
         for (int k = 0; k < BPT; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n) {
+                if (CT == ABIE_CTRL_PLAIN ? (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n) : (bool)E.slot_massive[k]) {
                     d0 = fmax(d0, fabs(y0[k][c]));
                     d1 = fmax(d1, fabs(a0[k][c]));
                 }
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
         for (int k = 0; k < BPT; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (E.slot_valid[k]) d2 = fmax(d2, fabs(acc[k][c] - a0[k][c]));
+                if (E.in_norm(k)) d2 = fmax(d2, fabs(acc[k][c] - a0[k][c]));
         wg_max2<NT>(d2, dummy, E.red);
         d2 = d2 / dt0;
         double dt1;
@@ -497,6 +497,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
         const double err = pow_pos(estim / kEpsb, 1. / 7);
         const double dtreq = h / err;
         const double h_att = h, t_att = t;
+        const int rej_before = rejects;
         bool accepted = false;
         bool finished = false;
         double h_next_plain;
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
         double q = dtreq / h_att;  // refine_bs ratio of the reference
 
         // the learned proposer (after accepted steps only)
-        if (CT == ABIE_CTRL_LEARNED && accepted && !finished && A.mlp != nullptr) {
+        if (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr) {
             if (tid < 64) {
                 float x[ABIE_MLP_IN];
                 const double eps = 1e-300;
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
                 x[5] = (float)log10(estim + eps);
                 x[6] = (float)log10((double)nm);
                 x[7] = (float)log10((double)(n - nm) + 1.0);
-                x[8] = (float)log10(dtreq / h_att);
+                x[8] = (float)log10(1.0 + (double)rej_before);
                 x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
                 x[10] = (float)ctl->prev_pc / 12.0f;
                 x[11] = 1.0f;
@@ -552,12 +553,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
             // keep the reference's growth / shrink bounds
             hp = fmin(hp, h_att / kFac);
             hp = fmax(hp, h_att * kFac);
+            // the refine_bs ratio follows the proposal: the reference's q
+            // scaled by the factor the proposer applied to its step
+            q = q * (hp / h_next_plain);
             h_next = hp;
-            q = hp / h_att;
             n_pu += 1;
         }
         if (prop_pending && !accepted) n_pr += 1;
-        prop_pending = (CT == ABIE_CTRL_LEARNED && accepted && !finished && A.mlp != nullptr);
+        prop_pending = (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr);
 
         // step log
         if (A.log != nullptr && tid == 0) {
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
             if ((long long)slot < A.log_cap) {
                 double *r = A.log + slot * ABIE_STEPLOG_FIELDS;
                 r[0] = sys; r[1] = t_att; r[2] = h_att; r[3] = estim; r[4] = err; r[5] = dtreq;
-                r[6] = accepted; r[7] = pc_used; r[8] = rejects; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
+                r[6] = accepted; r[7] = pc_used; r[8] = rej_before; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
                 r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = n - nm; r[15] = h_next;
             }
         }
@@ -576,7 +579,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
                 ctl->prev_dtreq = dtreq;
                 ctl->prev_pc = pc_used;
             }
-            if (!finished) {
+            {
+                // also on the last step: the state stays ready to continue
                 const KTab *KR = launder();
 #pragma unroll
                 for (int k = 0; k < BPT; ++k)
@@ -618,10 +622,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
                         long long nce = ctl->n_ce, nco = ctl->n_coll;
                         double *rce = A.ring_ce ? A.ring_ce + (long long)sys * 4 * A.max_ce : nullptr;
                         double *rco = A.ring_coll ? A.ring_coll + (long long)sys * 4 * A.max_coll : nullptr;
-                        for (int jo = 0; jo < n; ++jo) {
-                            const int je = A.inv[b0 + jo];
-                            for (int ko = jo + 1; ko < n; ++ko) {
-                                const int ke = A.inv[b0 + ko];
+                        for (int q1 = 0; q1 < n; ++q1) {
+                            const int je = A.inv[b0 + q1];
+                            const double jo = A.orig[b0 + je];
+                            for (int q2 = q1 + 1; q2 < n; ++q2) {
+                                const int ke = A.inv[b0 + q2];
+                                const double ko = A.orig[b0 + ke];
                                 const double dx = sx[b0 + je] - sx[b0 + ke], dyy = sy[b0 + je] - sy[b0 + ke],
                                              dz = sz[b0 + je] - sz[b0 + ke];
                                 const double r2 = dx * dx + dyy * dyy + dz * dz;
@@ -661,10 +667,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
                     }
                 }
             }
-            if (finished) {
-                status = ST_DONE;
-                break;
-            }
         }
         h = h_next;
         if (h < kHmin) {
@@ -673,6 +675,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
                        kHmin, t);
             warned = 1;
             h = kHmin;
+        }
+        if (finished) {
+            status = ST_DONE;
+            break;
         }
     }
 

@@ -57,9 +57,9 @@ struct DBuf {
 };
 
 static int device_cu_count(int dev) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
-    return prop.multiProcessorCount;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
+    return v;
 }
 
 static int force_mode_from_env() {
@@ -97,9 +97,11 @@ struct abie_ensemble {
     double ce_dist = 0.0;
     int max_ce = 0, max_coll = 0;
     int max_attempts = 256;
+    int n_units = 0;                   // work units (a system, or a slice of one)
     std::vector<int> offsets;          // [n_sys+1] original layout
-    std::vector<int> eng2orig;         // engine global index -> original global index
-    std::vector<SysDesc> desc;
+    std::vector<long long> eng2orig;   // engine index -> original global index (-1: replica)
+    std::vector<int> unit_sys, sys_unit0;
+    std::vector<SysDesc> desc;         // per unit
     std::vector<int> order;            // grouped by cfg
     int group_off[kNumCfg + 1] = {0};
     hipStream_t streams[kNumCfg] = {};
@@ -119,47 +121,89 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
                      const double *r, const double *ext) {
     const int S = e->n_sys;
     e->offsets.assign(offsets, offsets + S + 1);
-    e->total = offsets[S];
-    const long long T = e->total;
-    e->desc.resize(S);
-    e->eng2orig.resize(T);
-    std::vector<double> x(T), y(T), z(T), vx(T), vy(T), vz(T), gm(T), mm(T), rad(T), ex;
-    std::vector<int> orig(T), inv(T);
-    std::vector<unsigned char> del(T);
-    if (ext) ex.resize(3 * T);
-    std::vector<int> perm;
+    const int cap = kCfgs[kNumCfg - 1].nt * kCfgs[kNumCfg - 1].bpt;
+    // units: a system that fits one workgroup is one unit; with the
+    // massive-only controllers a larger system is split into slices that
+    // each carry every massive body plus a share of the test particles (the
+    // slices evolve the massive bodies identically, so they stay in step).
+    struct Unit {
+        int sys, nm, t0, t1;  // test-particle range [t0, t1) in engine order
+    };
+    std::vector<Unit> units;
+    std::vector<std::vector<int>> perms(S);
+    std::vector<int> nms(S);
     for (int s = 0; s < S; ++s) {
         const int o = offsets[s], n = offsets[s + 1] - offsets[s];
         if (n <= 0) return fail(ABIE_HIP_EINVAL, "system %d has no bodies", s);
-        const int nm = engine_order(m + o, n, perm);
-        const int cfg = pick_cfg(n, nm);
-        if (cfg < 0)
-            return fail(ABIE_HIP_ETOOBIG, "system %d: %d bodies / %d massive exceeds the engine (max %d bodies, %d massive)",
-                        s, n, nm, kCfgs[kNumCfg - 1].nt * kCfgs[kNumCfg - 1].bpt, kCfgs[kNumCfg - 1].nt);
-        e->desc[s] = SysDesc{o, n, nm, cfg};
+        const int nm = engine_order(m + o, n, perms[s]);
+        nms[s] = nm;
+        const int nt = n - nm;
+        if (pick_cfg(n, nm) >= 0) {
+            units.push_back({s, nm, nm, n});
+        } else if (e->ctrl != ABIE_CTRL_PLAIN && nm <= kCfgs[kNumCfg - 1].nt && cap - nm >= 64) {
+            const int per = cap - nm;
+            const int k = (nt + per - 1) / per;
+            const int chunk = (nt + k - 1) / k;
+            for (int q = 0; q < k; ++q) units.push_back({s, nm, nm + q * chunk, std::min(n, nm + (q + 1) * chunk)});
+        } else {
+            return fail(ABIE_HIP_ETOOBIG,
+                        "system %d: %d bodies / %d massive exceeds one workgroup (max %d bodies, %d massive); the "
+                        "massive-only controllers split larger systems",
+                        s, n, nm, cap, kCfgs[kNumCfg - 1].nt);
+        }
+    }
+    const int U = (int)units.size();
+    e->n_units = U;
+    long long T = 0;
+    for (auto &u : units) T += u.nm + (u.t1 - u.t0);
+    e->total = T;
+    e->desc.resize(U);
+    e->unit_sys.resize(U);
+    e->sys_unit0.assign(S, -1);
+    e->eng2orig.assign(T, -1);
+    std::vector<double> x(T), y(T), z(T), vx(T), vy(T), vz(T), gm(T), mm(T), rad(T), ex;
+    std::vector<int> orig(T), inv(T, 0);
+    std::vector<unsigned char> del(T);
+    if (ext) ex.resize(3 * T);
+    long long g0 = 0;
+    for (int u = 0; u < U; ++u) {
+        const Unit &un = units[u];
+        const int s = un.sys, o = offsets[s];
+        const bool first = e->sys_unit0[s] < 0;
+        if (first) e->sys_unit0[s] = u;
+        e->unit_sys[u] = s;
+        const int n = un.nm + (un.t1 - un.t0);
+        e->desc[u] = SysDesc{(int)g0, n, un.nm, pick_cfg(n, un.nm)};
         for (int p = 0; p < n; ++p) {
-            const int i = perm[p];
-            const long long g = o + p, gi = o + i;
-            e->eng2orig[g] = (int)gi;
+            const int ep = p < un.nm ? p : un.t0 + (p - un.nm);  // engine position in the whole system
+            const int i = perms[s][ep];
+            const long long g = g0 + p, gi = o + i;
+            if (first || p >= un.nm) e->eng2orig[g] = gi;
             x[g] = pos[3 * gi]; y[g] = pos[3 * gi + 1]; z[g] = pos[3 * gi + 2];
             vx[g] = vel[3 * gi]; vy[g] = vel[3 * gi + 1]; vz[g] = vel[3 * gi + 2];
             mm[g] = m[gi];
-            gm[g] = (p < nm) ? e->G * m[gi] : 0.0;
+            gm[g] = (p < un.nm) ? e->G * m[gi] : 0.0;
             rad[g] = r ? r[gi] : 0.0;
             del[g] = m[gi] < 0.0;
             orig[g] = i;
-            inv[o + i] = p;
             if (ext)
                 for (int c = 0; c < 3; ++c) ex[c * T + g] = ext[3 * gi + c];
         }
+        // the unit's bodies in ascending original index (the order of the
+        // reference's event loop, common.c:119-123)
+        std::vector<int> byorig(n);
+        std::iota(byorig.begin(), byorig.end(), 0);
+        std::sort(byorig.begin(), byorig.end(), [&](int a, int b) { return orig[g0 + a] < orig[g0 + b]; });
+        for (int q = 0; q < n; ++q) inv[g0 + q] = byorig[q];
+        g0 += n;
     }
     // LPT order inside each configuration group (largest work first)
     e->order.clear();
     for (int c = 0; c < kNumCfg; ++c) {
         e->group_off[c] = (int)e->order.size();
         std::vector<int> grp;
-        for (int s = 0; s < S; ++s)
-            if (e->desc[s].cfg == c) grp.push_back(s);
+        for (int u = 0; u < U; ++u)
+            if (e->desc[u].cfg == c) grp.push_back(u);
         std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) {
             const double wa = (double)e->desc[a].n * e->desc[a].nm, wb = (double)e->desc[b].n * e->desc[b].nm;
             return wa > wb;
@@ -168,10 +212,10 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     }
     e->group_off[kNumCfg] = (int)e->order.size();
 
-    HIP_TRY(e->d_sys.reserve(S));
-    HIP_TRY(e->d_order.reserve(S));
-    HIP_TRY(e->d_ctl.reserve(S));
-    HIP_TRY(e->d_energy.reserve(S));
+    HIP_TRY(e->d_sys.reserve(U));
+    HIP_TRY(e->d_order.reserve(U));
+    HIP_TRY(e->d_ctl.reserve(U));
+    HIP_TRY(e->d_energy.reserve(U));
     for (auto *b : {&e->d_x, &e->d_y, &e->d_z, &e->d_vx, &e->d_vy, &e->d_vz, &e->d_gm, &e->d_m, &e->d_rad})
         HIP_TRY(b->reserve(T));
     HIP_TRY(e->d_orig.reserve(T));
@@ -194,15 +238,15 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     HIP_TRY(hipMemcpy(e->d_orig.p, orig.data(), T * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_inv.p, inv.data(), T * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_del.p, del.data(), T, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->d_sys.p, e->desc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->d_order.p, e->order.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_sys.p, e->desc.data(), U * sizeof(SysDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_order.p, e->order.data(), U * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(e->d_state.p, 0, (size_t)kStateDoubles * T * sizeof(double)));
-    std::vector<SysCtl> ctl(S);
+    std::vector<SysCtl> ctl(U);
     for (auto &c : ctl) {
         memset(&c, 0, sizeof c);
         c.status = ST_RUNNING;
     }
-    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), S * sizeof(SysCtl), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
     for (int c = 0; c < kNumCfg; ++c)
         if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
     return 0;
@@ -212,12 +256,12 @@ static int ens_rings(abie_ensemble *e, int max_ce, int max_coll) {
     e->max_ce = max_ce;
     e->max_coll = max_coll;
     if (max_ce > 0) {
-        HIP_TRY(e->d_ring_ce.reserve((size_t)4 * max_ce * e->n_sys));
-        HIP_TRY(hipMemset(e->d_ring_ce.p, 0, (size_t)4 * max_ce * e->n_sys * sizeof(double)));
+        HIP_TRY(e->d_ring_ce.reserve((size_t)4 * max_ce * e->n_units));
+        HIP_TRY(hipMemset(e->d_ring_ce.p, 0, (size_t)4 * max_ce * e->n_units * sizeof(double)));
     }
     if (max_coll > 0) {
-        HIP_TRY(e->d_ring_coll.reserve((size_t)4 * max_coll * e->n_sys));
-        HIP_TRY(hipMemset(e->d_ring_coll.p, 0, (size_t)4 * max_coll * e->n_sys * sizeof(double)));
+        HIP_TRY(e->d_ring_coll.reserve((size_t)4 * max_coll * e->n_units));
+        HIP_TRY(hipMemset(e->d_ring_coll.p, 0, (size_t)4 * max_coll * e->n_units * sizeof(double)));
     }
     return 0;
 }
@@ -254,21 +298,21 @@ static GR15Args ens_args(abie_ensemble *e) {
 }
 
 static int ens_set_t_end(abie_ensemble *e, double t_end, const double *t_start) {
-    std::vector<SysCtl> ctl(e->n_sys);
-    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
-    for (int s = 0; s < e->n_sys; ++s) {
-        ctl[s].t_end = t_end;
-        if (t_start) ctl[s].t = t_start[s];
-        if (ctl[s].status == ST_DONE && ctl[s].t <= t_end) ctl[s].status = ST_RUNNING;
+    std::vector<SysCtl> ctl(e->n_units);
+    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+    for (int u = 0; u < e->n_units; ++u) {
+        ctl[u].t_end = t_end;
+        if (t_start) ctl[u].t = t_start[e->unit_sys[u]];
+        if (ctl[u].status == ST_DONE && ctl[u].t <= t_end) ctl[u].status = ST_RUNNING;
     }
-    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), e->n_sys * sizeof(SysCtl), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), e->n_units * sizeof(SysCtl), hipMemcpyHostToDevice));
     return 0;
 }
 
 // Launch every configuration group until all systems stop or time is up.
 static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<SysCtl> ctl(e->n_sys);
+    std::vector<SysCtl> ctl(e->n_units);
     int done = 0;
     HIP_TRY(hipSetDevice(e->dev));
     for (;;) {
@@ -281,7 +325,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
         }
         for (int c = 0; c < kNumCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
-        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
         bool any_running = false;
         for (auto &c : ctl) any_running |= (c.status == ST_RUNNING);
         if (!any_running) {
@@ -360,6 +404,7 @@ int abie_ensemble_get_state(abie_ensemble *e, double *pos, double *vel, double *
             HIP_TRY(hipMemcpy(c.data(), z.p, T * sizeof(double), hipMemcpyDeviceToHost));
             for (long long g = 0; g < T; ++g) {
                 const long long i = e->eng2orig[g];
+                if (i < 0) continue;
                 dst[3 * i] = a[g]; dst[3 * i + 1] = b[g]; dst[3 * i + 2] = c[g];
             }
             return 0;
@@ -368,11 +413,15 @@ int abie_ensemble_get_state(abie_ensemble *e, double *pos, double *vel, double *
         if (vel && put(vel, e->d_vx, e->d_vy, e->d_vz)) return ABIE_HIP_EHIP;
     }
     if (t || status) {
-        std::vector<SysCtl> ctl(e->n_sys);
-        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        std::vector<SysCtl> ctl(e->n_units);
+        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
         for (int s = 0; s < e->n_sys; ++s) {
-            if (t) t[s] = ctl[s].t;
-            if (status) status[s] = ctl[s].status == ST_DONE ? 0 : ctl[s].status;
+            const SysCtl &c = ctl[e->sys_unit0[s]];
+            if (t) t[s] = c.t;
+            int st = c.status == ST_DONE ? 0 : c.status;
+            for (int u = 0; u < e->n_units; ++u)  // any slice's event or error wins
+                if (e->unit_sys[u] == s && ctl[u].status != ST_DONE && ctl[u].status != ST_RUNNING) st = ctl[u].status;
+            if (status) status[s] = st;
         }
     }
     return 0;
@@ -380,21 +429,24 @@ int abie_ensemble_get_state(abie_ensemble *e, double *pos, double *vel, double *
 
 int abie_ensemble_get_counters(abie_ensemble *e, long long *cnt) {
     if (!e || !cnt) return fail(ABIE_HIP_EINVAL, "ensemble_get_counters: bad arguments");
-    std::vector<SysCtl> ctl(e->n_sys);
-    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_sys * sizeof(SysCtl), hipMemcpyDeviceToHost));
+    std::vector<SysCtl> ctl(e->n_units);
+    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
     for (int s = 0; s < e->n_sys; ++s) {
+        const SysCtl &u = ctl[e->sys_unit0[s]];
         long long *c = cnt + 6 * s;
-        c[0] = ctl[s].n_evals; c[1] = ctl[s].n_attempts; c[2] = ctl[s].n_accepted;
-        c[3] = ctl[s].n_pc; c[4] = ctl[s].n_prop_used; c[5] = ctl[s].n_prop_rej;
+        c[0] = u.n_evals; c[1] = u.n_attempts; c[2] = u.n_accepted;
+        c[3] = u.n_pc; c[4] = u.n_prop_used; c[5] = u.n_prop_rej;
     }
     return 0;
 }
 
 int abie_ensemble_energy(abie_ensemble *e, double *energy) {
     if (!e || !energy) return fail(ABIE_HIP_EINVAL, "ensemble_energy: bad arguments");
-    HIP_TRY(launch_energy(e->d_sys.p, e->n_sys, e->d_x.p, e->d_y.p, e->d_z.p, e->d_vx.p, e->d_vy.p, e->d_vz.p,
+    HIP_TRY(launch_energy(e->d_sys.p, e->n_units, e->d_x.p, e->d_y.p, e->d_z.p, e->d_vx.p, e->d_vy.p, e->d_vz.p,
                           e->d_m.p, e->G, e->d_energy.p, 0));
-    HIP_TRY(hipMemcpy(energy, e->d_energy.p, e->n_sys * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> eu(e->n_units);
+    HIP_TRY(hipMemcpy(eu.data(), e->d_energy.p, e->n_units * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->n_sys; ++s) energy[s] = eu[e->sys_unit0[s]];
     return 0;
 }
 
@@ -592,7 +644,14 @@ int initialize_code(double G, double C, size_t N_MAX, size_t MAX_N_CE, size_t MA
     if (g.ext.size() < 3 * N_MAX) g.ext.assign(3 * N_MAX, 0.0);
     g.buf_ce.assign(4 * MAX_N_CE, 0.0);
     g.buf_coll.assign(4 * MAX_N_COLLISIONS, 0.0);
+    // a fresh simulation: counters, clock and extra forces start empty (the
+    // reference initialises once per process, so it never re-enters here)
     g.n_ce = 0;
+    g.n_coll = 0;
+    g.t = 0.0;
+    g.n = 0;
+    g.ext_on = false;
+    std::fill(g.ext.begin(), g.ext.end(), 0.0);
     if (!g.mode_set) g.force_mode = force_mode_from_env();
     g.inited = true;
     printf("Initialized.\n");
@@ -607,6 +666,8 @@ int finalize_code(void) {
     g.stream = nullptr;
     g.inited = false;
     g.n = 0;
+    g.ext_on = false;
+    g.ext.clear();
     return 0;
 }
 

@@ -151,6 +151,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the measurement, copy a buffer of this size 5 times "
+                         "(reference traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
@@ -209,6 +212,14 @@ def main():
         log("mixed/exact: %s %s" % (extras["config1a_mixed"], extras["config1a_exact"]))
         extras["gr15_dropin_config0_10yr"] = gr15_dropin(10.0)
         log("gr15 drop-in: %s" % extras["gr15_dropin_config0_10yr"])
+    if args.copy_calibration_mib > 0:
+        nel = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        extras["copy_calibration_bytes"] = nel * 8
     out["extras"] = extras
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -152,6 +152,11 @@ double orc_energy(const orc_sim *s)
     return e;
 }
 
+static double orc_pow(const orc_sim *s, double x, double y)
+{
+    return s->pow_fn ? s->pow_fn(x, y) : pow(x, y);
+}
+
 /* integrator_gauss_radau15.c:9-54.  Note the two max-abs calls over only the
  * first n (not 3n) components (:23-24), kept as in the reference. */
 double orc_initial_time_step(orc_sim *s, const double *y0, const double *dy0)
@@ -181,7 +186,7 @@ double orc_initial_time_step(orc_sim *s, const double *y0, const double *dy0)
     d2 = d2 / dt0;
 
     if (fmax(d1, d2) <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
-    else dt1 = pow(0.01 / fmax(d1, d2), (1.0 / (p + 1)));
+    else dt1 = orc_pow(s, 0.01 / fmax(d1, d2), (1.0 / (p + 1)));
 
     free(f0); free(f1); free(y1); free(dy1);
     return fmin(100 * dt0, dt1);
@@ -324,7 +329,7 @@ int orc_gr15(orc_sim *s, double t_start, double t_end, double dt_user)
         orc_calculate_accelerations(s, y, dy, ddy);
 
         const double estim_b6 = orc_vector_max_abs(bs + 6 * dim, dim) / orc_vector_max_abs(ddy, dim);
-        const double err = pow(estim_b6 / ORC_EPSB, 1. / 7);
+        const double err = orc_pow(s, estim_b6 / ORC_EPSB, 1. / 7);
         const double dtreq = h / err;
         const double t_attempt = t, h_attempt = h;
         int accepted = 0;

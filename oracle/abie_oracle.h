@@ -55,6 +55,8 @@ typedef struct orc_sim {
     size_t log_cap, log_len;
     int skip_first_node;      /* 1: reuse a(y0) for node 0 instead of a new
                                  evaluation (bit-identical; saves 1 of 8)   */
+    double (*pow_fn)(double, double); /* NULL: libm pow (as the reference);
+                                 else used for the step-size exponents      */
 } orc_sim;
 
 /* common.c:39-77  ode_n_body_second_order */

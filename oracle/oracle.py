@@ -40,7 +40,28 @@ class OrcSim(ctypes.Structure):
                 ("n_accepted", ctypes.c_longlong), ("n_pc_iters", ctypes.c_longlong),
                 ("n_hmin", ctypes.c_longlong),
                 ("log", ctypes.POINTER(Attempt)), ("log_cap", ctypes.c_size_t),
-                ("log_len", ctypes.c_size_t), ("skip_first_node", ctypes.c_int)]
+                ("log_len", ctypes.c_size_t), ("skip_first_node", ctypes.c_int),
+                ("pow_fn", ctypes.c_void_p)]
+
+
+POWFN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_double)
+
+
+def _cr_pow(x, y):
+    """x**y rounded to nearest from a 40-digit mpmath evaluation."""
+    import math
+    import mpmath as mp
+    if x != x or x < 0:
+        return math.nan
+    if x == 0.0:
+        return 0.0
+    if math.isinf(x):
+        return x
+    with mp.workdps(40):
+        return float(mp.power(mp.mpf(x), mp.mpf(y)))
+
+
+CR_POW = POWFN(_cr_pow)
 
 
 def build():
@@ -107,7 +128,7 @@ class Sim:
 
     def __init__(self, pos, vel, m, r=None, G=4 * np.pi ** 2, C=0.0,
                  max_ce=1, max_coll=1, ce_dist=0.0, ext_acc=None,
-                 log_cap=0, skip_first_node=False):
+                 log_cap=0, skip_first_node=False, cr_pow=False):
         n = len(m)
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1).copy()
         self.vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1).copy()
@@ -130,6 +151,10 @@ class Sim:
             s.log = ctypes.cast(self.log, ctypes.POINTER(Attempt))
         s.log_cap = log_cap
         s.skip_first_node = 1 if skip_first_node else 0
+        # cr_pow: correctly rounded step-size exponents (glibc's pow is not
+        # correctly rounded for ~0.1% of x**(1/7)); the device engine rounds
+        # them correctly, so this is the oracle the device must match bitwise
+        s.pow_fn = ctypes.cast(CR_POW, ctypes.c_void_p).value if cr_pow else None
         self.s = s
 
     @property

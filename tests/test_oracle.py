@@ -240,3 +240,30 @@ def test_energy_reference_and_closed_form(orc, golden):
     vel = np.array([[0, -v * m2 / (m1 + m2), 0], [0, v * m1 / (m1 + m2), 0]])
     s = orc.Sim(pos, vel, np.array([m1, m2]))
     assert s.energy() == pytest.approx(-G * m1 * m2 / (2 * a), rel=1e-13)
+
+
+def test_cr_pow_oracle_agrees_on_golden_paths(orc, golden):
+    """glibc's pow is not correctly rounded for ~0.1% of x**(1/7); the oracle
+    can use a correctly rounded pow (mpmath) instead.  On the golden runs the
+    two give the same trajectories, bit for bit."""
+    d = golden("gr15")
+    for pre in ("small", "c0"):
+        m = d[pre + "_m"]
+        a = orc.Sim(d[pre + "_pos0"], d[pre + "_vel0"], m)
+        b = orc.Sim(d[pre + "_pos0"], d[pre + "_vel0"], m, cr_pow=True)
+        a.integrate(1.0, 0.0)
+        b.integrate(1.0, 0.0)
+        assert a.t == b.t and same_bits(a.pos, b.pos)
+
+
+def test_glibc_pow_is_not_always_correctly_rounded():
+    """Documents why the device rounds the step-size exponent itself: a
+    known input where glibc's pow(x, 1/7) is one ulp off."""
+    import math
+    import mpmath as mp
+    x = 2.8510539928770382
+    with mp.workdps(50):
+        cr = float(mp.power(mp.mpf(x), mp.mpf(1.0 / 7)))
+    from oracle import oracle as O
+    assert O._cr_pow(x, 1.0 / 7) == cr
+    assert math.pow(x, 1.0 / 7) in (cr, math.nextafter(cr, 0), math.nextafter(cr, 2))
