@@ -62,6 +62,9 @@ SIGNATURES = {
     "abie_ensemble_set_steplog": (c_i, [c_vp, ctypes.c_longlong]),
     "abie_ensemble_get_steplog": (c_i, [c_vp, c_dp, c_llp]),
     "abie_ensemble_set_proposer": (c_i, [c_vp, c_fp, c_i]),
+    "abie_gr15_num_configs": (c_i, []),
+    "abie_gr15_config": (c_i, [c_i, c_ip, c_ip]),
+    "abie_gr15_set_tuning": (c_i, [c_i, c_ip, c_ip]),
     "abie_mlp_forward": (c_i, [c_vp, c_vp, c_vp, c_i, c_vp]),
     "abie_mlp_loss_grad": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp]),
     "abie_mlp_adam": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -102,7 +105,24 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        _apply_tuning(lib)
     return _lib
+
+
+TUNING_PATH = os.path.join(HERE, "data", "gr15_tuning.json")
+
+
+def _apply_tuning(lib):
+    """Use the autotuner's rules table (abie_amd/autotune.py) when present."""
+    import json
+    try:
+        with open(os.environ.get("ABIE_HIP_TUNING", TUNING_PATH)) as f:
+            rules = json.load(f)["rules"]
+    except (OSError, ValueError, KeyError):
+        return
+    mx = (ctypes.c_int * len(rules))(*[int(r[0]) for r in rules])
+    cf = (ctypes.c_int * len(rules))(*[int(r[1]) for r in rules])
+    lib.abie_gr15_set_tuning(len(rules), mx, cf)
 
 
 def check(rc, what="libabie_hip"):

@@ -98,9 +98,20 @@ __device__ __forceinline__ void tile_sum(const double *__restrict__ tx, const do
                                          const double (&px)[SPT], const double (&py)[SPT],
                                          const double (&pz)[SPT], double (&ax)[SPT], double (&ay)[SPT],
                                          double (&az)[SPT], Coef kc) {
-    if (!CHECK && cnt == BLOCK) {
-#pragma unroll 4
-        for (int k = 0; k < BLOCK; ++k) {
+    if (!CHECK) {
+        // groups of 4 sources, fully unrolled with the SPT sinks: 4*SPT
+        // independent chains in flight; then the remainder
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double sx = tx[k + u], sy = ty[k + u], sz = tz[k + u], g = tg[k + u];
+#pragma unroll
+                for (int s = 0; s < SPT; ++s)
+                    Interact<MODE>::run(px[s], py[s], pz[s], sx, sy, sz, g, ax[s], ay[s], az[s], kc);
+            }
+        }
+        for (; k < cnt; ++k) {
             const double sx = tx[k], sy = ty[k], sz = tz[k], g = tg[k];
 #pragma unroll
             for (int s = 0; s < SPT; ++s) Interact<MODE>::run(px[s], py[s], pz[s], sx, sy, sz, g, ax[s], ay[s], az[s], kc);

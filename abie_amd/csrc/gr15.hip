@@ -1,37 +1,89 @@
 // gr15.hip -- configuration table, dispatch, energy and event kernels of
 // the GR15 engine (the engine itself is gr15_impl.hpp).
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "devmath.hpp"
 #include "gr15.hpp"
 
 namespace abie {
 
-const GR15Cfg kCfgs[kNumCfg] = {{64, 1}, {128, 1}, {256, 1}, {512, 1}, {512, 2}};
+const GR15Cfg kCfgs[kNumCfg] = {{64, 1},  {128, 1}, {256, 1}, {256, 2}, {512, 1}, {256, 4},
+                                 {512, 2}, {1024, 1}, {512, 3}, {512, 4}, {512, 5}};
+
+bool cfg_fits(int c, int n, int nm) {
+    return c >= 0 && c < kNumCfg && n <= kCfgs[c].nt * kCfgs[c].bpt && nm <= src_cap(kCfgs[c].nt);
+}
+
+int max_bodies_one_group() {
+    int m = 0;
+    for (int c = 0; c < kNumCfg; ++c) m = std::max(m, kCfgs[c].nt * kCfgs[c].bpt);
+    return m;
+}
+
+// Default rules: the smallest group that holds the system, favouring
+// configurations whose state fits the register file.
+static std::vector<std::pair<int, int>> g_rules = {{64, 0}, {128, 1}, {256, 2}, {512, 3}, {1024, 5},
+                                                   {1536, 8}, {2048, 9}, {2560, 10}};
+
+int set_tuning(int n_rules, const int *max_n, const int *cfg) {
+    if (n_rules <= 0 || !max_n || !cfg) return -1;
+    std::vector<std::pair<int, int>> r;
+    for (int i = 0; i < n_rules; ++i) {
+        if (cfg[i] < 0 || cfg[i] >= kNumCfg) return -1;
+        r.push_back({max_n[i], cfg[i]});
+    }
+    g_rules = r;
+    return 0;
+}
 
 int pick_cfg(int n, int nm) {
+    for (auto &r : g_rules)
+        if (n <= r.first && cfg_fits(r.second, n, nm)) return r.second;
+    // no rule: the smallest configuration that fits
+    int best = -1;
     for (int c = 0; c < kNumCfg; ++c)
-        if (n <= kCfgs[c].nt * kCfgs[c].bpt && nm <= kCfgs[c].nt) return c;
-    return -1;
+        if (cfg_fits(c, n, nm) && (best < 0 || kCfgs[c].nt * kCfgs[c].bpt < kCfgs[best].nt * kCfgs[best].bpt))
+            best = c;
+    return best;
 }
 
 // LDS: sources [2][NT] x {x, y, z, gm}, then 128 doubles of scratch:
 // [0, 32) reductions, [40, 48) broadcast flags, [64, 128) proposer hidden layer.
-size_t gr15_lds_bytes(int cfg) { return (size_t)2 * kCfgs[cfg].nt * 4 * sizeof(double) + 128 * sizeof(double); }
+size_t gr15_lds_bytes(int cfg) { return (size_t)2 * src_cap(kCfgs[cfg].nt) * 4 * sizeof(double) + 128 * sizeof(double); }
 
 
-hipError_t launch_gr15_fp64(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
-hipError_t launch_gr15_exact(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
-hipError_t launch_gr15_mixed(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st);
+// one translation unit per (force mode, controller) pair, compiled in parallel
+hipError_t launch_gr15_fp64_plain(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_fp64_massive(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_fp64_learned(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_exact_plain(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_mixed_massive(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_mixed_learned(int, const GR15Args &, int, size_t, hipStream_t);
+
+bool gr15_combo_supported(int fm, int ctrl) {
+    if (fm == ABIE_FORCE_FP64) return ctrl >= 0 && ctrl <= 2;
+    if (fm == ABIE_FORCE_FP64_EXACT) return ctrl == ABIE_CTRL_PLAIN;
+    if (fm == ABIE_FORCE_MIXED) return ctrl == ABIE_CTRL_MASSIVE || ctrl == ABIE_CTRL_LEARNED;
+    return false;
+}
 
 hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st) {
     if (nb <= 0) return hipSuccess;
     if (cfg < 0 || cfg >= kNumCfg) return hipErrorInvalidValue;
     const size_t lds = gr15_lds_bytes(cfg);
-    switch (fm) {
-        case ABIE_FORCE_FP64: return launch_gr15_fp64(cfg, ctrl, a, nb, lds, st);
-        case ABIE_FORCE_FP64_EXACT: return launch_gr15_exact(cfg, ctrl, a, nb, lds, st);
-        case ABIE_FORCE_MIXED: return launch_gr15_mixed(cfg, ctrl, a, nb, lds, st);
-        default: return hipErrorInvalidValue;
+    if (fm == ABIE_FORCE_FP64) {
+        if (ctrl == ABIE_CTRL_PLAIN) return launch_gr15_fp64_plain(cfg, a, nb, lds, st);
+        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_fp64_massive(cfg, a, nb, lds, st);
+        return launch_gr15_fp64_learned(cfg, a, nb, lds, st);
     }
+    if (fm == ABIE_FORCE_FP64_EXACT && ctrl == ABIE_CTRL_PLAIN) return launch_gr15_exact_plain(cfg, a, nb, lds, st);
+    if (fm == ABIE_FORCE_MIXED) {
+        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_mixed_massive(cfg, a, nb, lds, st);
+        if (ctrl == ABIE_CTRL_LEARNED) return launch_gr15_mixed_learned(cfg, a, nb, lds, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 // check_collisions_close_encounters (common.c:116-158) for one system: all

@@ -65,11 +65,19 @@ struct GR15Args {
 struct GR15Cfg {
     int nt, bpt;
 };
-constexpr int kNumCfg = 5;
+// Massive bodies one group holds in its LDS source list (<= 64 KB of LDS).
+constexpr int src_cap(int nt) { return nt < 1008 ? nt : 1008; }
+constexpr int kNumCfg = 11;
 extern const GR15Cfg kCfgs[kNumCfg];
-int pick_cfg(int n, int nm);  // -1 if no configuration fits
+int pick_cfg(int n, int nm);  // by the tuning rules; -1 if no configuration fits
+bool cfg_fits(int cfg, int n, int nm);
+int max_bodies_one_group();
+// Tuning rules: a system of n bodies uses the first rule with n <= max_n whose
+// configuration fits it.
+int set_tuning(int n_rules, const int *max_n, const int *cfg);
 size_t gr15_lds_bytes(int cfg);
 
+bool gr15_combo_supported(int force_mode, int ctrl);
 hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st);
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
                          int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *ret, hipStream_t st);

@@ -1,0 +1,9 @@
+// gr15_exact_plain.hip -- GR15 engine for ABIE_FORCE_FP64_EXACT forces under the ABIE_CTRL_PLAIN controller
+// (one translation unit per pair so that they compile in parallel).
+#include "gr15_impl.hpp"
+
+namespace abie {
+hipError_t launch_gr15_exact_plain(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    return launch_gr15_fm_ct<ABIE_FORCE_FP64_EXACT, ABIE_CTRL_PLAIN>(cfg, a, nb, lds, st);
+}
+}  // namespace abie

@@ -26,9 +26,6 @@
 //  reference bit for bit (tests/test_gpu_gr15.py).
 #pragma once
 
-#ifndef ABIE_GR15_WPE
-#define ABIE_GR15_WPE 2  // waves per SIMD the engine is compiled for (256 VGPRs)
-#endif
 
 #include "devmath.hpp"
 #include "gr15.hpp"
@@ -253,7 +250,7 @@ struct Engine {
     bool wave_massive;
 
     __device__ Engine(const GR15Args &a, int s, const SysDesc &d, double *lds)
-        : A(a), sys(s), body0(d.body0), n(d.n), nm(d.nm), src(lds), red(lds + 2 * NT * 4) {
+        : A(a), sys(s), body0(d.body0), n(d.n), nm(d.nm), src(lds), red(lds + 2 * src_cap(NT) * 4) {
         kc = Coef2{1.5, 1.875};
         asm volatile("" : "+v"(kc.c1), "+v"(kc.c2));
         wave_massive = ((int)(threadIdx.x & ~63) < nm);
@@ -269,7 +266,7 @@ struct Engine {
 
     // accelerations at positions p for every owned body; one barrier
     __device__ void forces(const double (&p)[BPT][3], double (&acc)[BPT][3]) {
-        double *S = src + buf * NT * 4;
+        double *S = src + buf * src_cap(NT) * 4;
         buf ^= 1;
 #pragma unroll
         for (int k = 0; k < BPT; ++k) {
@@ -344,8 +341,11 @@ __device__ float mlp_eval(const float *__restrict__ w, const float *x, float *ld
     return o + b3[0];
 }
 
-template <int NT, int BPT, int FM, int CT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WPE))) void gr15_kernel(GR15Args A) {
+// WPE: waves per SIMD the register allocation targets (512 / WPE registers
+// per lane, AGPRs included): 1 lets a 256-thread group keep 4 bodies per
+// thread in VGPRs + AGPRs, 2 is the most a 512-thread group can have.
+template <int NT, int BPT, int FM, int CT, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void gr15_kernel(GR15Args A) {
     extern __shared__ double lds[];
     const int sys = A.order[blockIdx.x];
     const SysDesc d = A.sys[sys];
@@ -724,30 +724,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ABIE_GR15_WP
 }
 
 
-template <int NT, int BPT, int FM>
-static hipError_t launch_ct(int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
-    switch (ctrl) {
-        case ABIE_CTRL_PLAIN:
-            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_PLAIN>), dim3(nb), dim3(NT), lds, st, a);
-            break;
-        case ABIE_CTRL_MASSIVE:
-            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_MASSIVE>), dim3(nb), dim3(NT), lds, st, a);
-            break;
-        default:
-            hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, ABIE_CTRL_LEARNED>), dim3(nb), dim3(NT), lds, st, a);
-            break;
-    }
+template <int NT, int BPT, int FM, int CT, int WPE>
+static hipError_t launch_one(const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, CT, WPE>), dim3(nb), dim3(NT), lds, st, a);
     return hipGetLastError();
 }
 
-template <int FM>
-hipError_t launch_gr15_fm(int cfg, int ctrl, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+// Instantiates every configuration of kCfgs for one (force mode, controller).
+template <int FM, int CT>
+hipError_t launch_gr15_fm_ct(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
     switch (cfg) {
-        case 0: return launch_ct<64, 1, FM>(ctrl, a, nb, lds, st);
-        case 1: return launch_ct<128, 1, FM>(ctrl, a, nb, lds, st);
-        case 2: return launch_ct<256, 1, FM>(ctrl, a, nb, lds, st);
-        case 3: return launch_ct<512, 1, FM>(ctrl, a, nb, lds, st);
-        case 4: return launch_ct<512, 2, FM>(ctrl, a, nb, lds, st);
+        case 0: return launch_one<64, 1, FM, CT, 2>(a, nb, lds, st);
+        case 1: return launch_one<128, 1, FM, CT, 2>(a, nb, lds, st);
+        case 2: return launch_one<256, 1, FM, CT, 2>(a, nb, lds, st);
+        case 3: return launch_one<256, 2, FM, CT, 1>(a, nb, lds, st);
+        case 4: return launch_one<512, 1, FM, CT, 2>(a, nb, lds, st);
+        case 5: return launch_one<256, 4, FM, CT, 1>(a, nb, lds, st);
+        case 6: return launch_one<512, 2, FM, CT, 2>(a, nb, lds, st);
+        case 7: return launch_one<1024, 1, FM, CT, 4>(a, nb, lds, st);
+        case 8: return launch_one<512, 3, FM, CT, 2>(a, nb, lds, st);
+        case 9: return launch_one<512, 4, FM, CT, 2>(a, nb, lds, st);
+        case 10: return launch_one<512, 5, FM, CT, 2>(a, nb, lds, st);
         default: return hipErrorInvalidValue;
     }
 }

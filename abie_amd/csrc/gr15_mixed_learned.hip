@@ -1,0 +1,9 @@
+// gr15_mixed_learned.hip -- GR15 engine for ABIE_FORCE_MIXED forces under the ABIE_CTRL_LEARNED controller
+// (one translation unit per pair so that they compile in parallel).
+#include "gr15_impl.hpp"
+
+namespace abie {
+hipError_t launch_gr15_mixed_learned(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    return launch_gr15_fm_ct<ABIE_FORCE_MIXED, ABIE_CTRL_LEARNED>(cfg, a, nb, lds, st);
+}
+}  // namespace abie

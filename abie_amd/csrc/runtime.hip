@@ -121,7 +121,9 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
                      const double *r, const double *ext) {
     const int S = e->n_sys;
     e->offsets.assign(offsets, offsets + S + 1);
-    const int cap = kCfgs[kNumCfg - 1].nt * kCfgs[kNumCfg - 1].bpt;
+    // bodies per slice when a system is split (massive-only controllers)
+    int cap = 1024;
+    if (const char *sl = getenv("ABIE_HIP_SLICE")) cap = std::max(128, atoi(sl));
     // units: a system that fits one workgroup is one unit; with the
     // massive-only controllers a larger system is split into slices that
     // each carry every massive body plus a share of the test particles (the
@@ -138,9 +140,10 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         const int nm = engine_order(m + o, n, perms[s]);
         nms[s] = nm;
         const int nt = n - nm;
-        if (pick_cfg(n, nm) >= 0) {
+        const bool slice = e->ctrl != ABIE_CTRL_PLAIN && n > cap && nm <= 512 && cap - nm >= 64;
+        if (!slice && pick_cfg(n, nm) >= 0) {
             units.push_back({s, nm, nm, n});
-        } else if (e->ctrl != ABIE_CTRL_PLAIN && nm <= kCfgs[kNumCfg - 1].nt && cap - nm >= 64) {
+        } else if (slice) {
             const int per = cap - nm;
             const int k = (nt + per - 1) / per;
             const int chunk = (nt + k - 1) / k;
@@ -149,7 +152,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
             return fail(ABIE_HIP_ETOOBIG,
                         "system %d: %d bodies / %d massive exceeds one workgroup (max %d bodies, %d massive); the "
                         "massive-only controllers split larger systems",
-                        s, n, nm, cap, kCfgs[kNumCfg - 1].nt);
+                        s, n, nm, max_bodies_one_group(), src_cap(1024));
         }
     }
     const int U = (int)units.size();
@@ -354,8 +357,10 @@ int abie_hip_set_device(int device) {
 int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, const double *pos, const double *vel,
                          const double *m, const double *r, double G, int force_mode, int controller) {
     if (!out || n_sys <= 0 || !offsets || !pos || !vel || !m) return fail(ABIE_HIP_EINVAL, "ensemble_create: bad arguments");
-    if (force_mode < 0 || force_mode > 2 || controller < 0 || controller > 2)
-        return fail(ABIE_HIP_EINVAL, "ensemble_create: bad mode %d / controller %d", force_mode, controller);
+    if (!gr15_combo_supported(force_mode, controller))
+        return fail(ABIE_HIP_EINVAL, "ensemble_create: force mode %d with controller %d is not built "
+                    "(exact forces run with the plain controller, mixed forces with the massive-only ones)",
+                    force_mode, controller);
     auto *e = new abie_ensemble();
     (void)hipGetDevice(&e->dev);
     e->n_sys = n_sys;
@@ -808,6 +813,20 @@ size_t check_collisions_close_encounters(const double *pos, const double *radii,
     g.n_ce = (size_t)cnt[0];
     g.n_coll = (size_t)cnt[1];
     return (size_t)ret;
+}
+
+int abie_gr15_num_configs(void) { return kNumCfg; }
+
+int abie_gr15_config(int cfg, int *threads, int *bodies_per_thread) {
+    if (cfg < 0 || cfg >= kNumCfg) return fail(ABIE_HIP_EINVAL, "no configuration %d", cfg);
+    if (threads) *threads = kCfgs[cfg].nt;
+    if (bodies_per_thread) *bodies_per_thread = kCfgs[cfg].bpt;
+    return 0;
+}
+
+int abie_gr15_set_tuning(int n_rules, const int *max_bodies, const int *cfg) {
+    if (set_tuning(n_rules, max_bodies, cfg) != 0) return fail(ABIE_HIP_EINVAL, "bad tuning rules");
+    return 0;
 }
 
 int abie_hip_set_force_mode(int mode) {

@@ -1,0 +1,122 @@
+"""Train the learned GR15 step proposer and measure what it gains.
+
+    python -m abie_amd.train_proposer [--systems 1024] [--years 20] [--out abie_amd/data/proposer.npz]
+
+1. Step logs: ``--systems`` seed systems (drawn like the ensemble workload:
+   massive U{3..64}, test particles U{0..2048}, seeds disjoint from the
+   benchmark's) are integrated by the device engine with the massive-only
+   controller and the step log on.
+2. Dataset: every accepted attempt that has a successor gives one sample
+   (features of the accepted step, target log10(dtreq_next / dtreq)).
+3. Training: the HIP kernels abie_mlp_loss_grad / abie_mlp_adam, on 90 % of
+   the systems; the other 10 % are held out.
+4. Margin: the output shift that maximises predicted step length per
+   attempt on the held-out samples (a rejected attempt costs its step).
+5. Evaluation: a fresh set of systems integrated with and without the
+   proposer; evaluations per simulated year, rejection rate, |dE/E|.
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import engine, ics, proposer
+
+G = 4 * np.pi ** 2
+TRAIN_SEED = 20261203 + 500000
+EVAL_SEED = 20261203 + 900000
+
+
+def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None):
+    systems = ics.ensemble(n_sys, seed=seed)
+    ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller="massive")
+    cap = cap or n_sys * 4000
+    ens.set_steplog(cap)
+    ens.run(years)
+    rec = ens.steplog()
+    ens.close()
+    return rec
+
+
+def choose_margin(pred, y, grid=np.linspace(0.0, 0.3, 31)):
+    """Margin that maximises expected step length per attempt: a proposal
+    dtreq*10^(p-m) is accepted when p - m <= y (the next dtreq ratio)."""
+    best, best_m = -1.0, 0.0
+    for m in grid:
+        prop = pred - m
+        ok = prop <= y
+        gain = np.mean(np.where(ok, 10.0 ** prop, 0.0))
+        if gain > best:
+            best, best_m = gain, m
+    return float(best_m), float(best)
+
+
+def evaluate(weights, n_sys, years, force_mode="fp64"):
+    out = {}
+    systems = ics.ensemble(n_sys, seed=EVAL_SEED)
+    for name, ctrl, w in (("massive", "massive", None), ("learned", "learned", weights)):
+        ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller=ctrl)
+        if w is not None:
+            ens.set_proposer(w)
+        e0 = ens.energy()
+        t0 = time.perf_counter()
+        ens.run(years)
+        wall = time.perf_counter() - t0
+        _, _, t, _ = ens.state()
+        cnt = ens.counters()
+        e1 = ens.energy()
+        ens.close()
+        out[name] = {"wall_s": wall, "system_yr_per_s": float(t.sum() / wall),
+                     "evals_per_yr": float(cnt[:, 0].sum() / t.sum()),
+                     "attempts": int(cnt[:, 1].sum()), "accepted": int(cnt[:, 2].sum()),
+                     "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1)),
+                     "proposals": int(cnt[:, 4].sum()), "proposals_rejected": int(cnt[:, 5].sum()),
+                     "max_dE_E": float(np.max(np.abs((e1 - e0) / e0)))}
+    out["evals_ratio"] = out["massive"]["evals_per_yr"] / out["learned"]["evals_per_yr"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--systems", type=int, default=1024)
+    ap.add_argument("--years", type=float, default=20.0)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--eval-systems", type=int, default=256)
+    ap.add_argument("--eval-years", type=float, default=10.0)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "proposer.npz"))
+    args = ap.parse_args()
+    t0 = time.perf_counter()
+    rec = collect_logs(args.systems, args.years, TRAIN_SEED)
+    print("step log: %d attempts from %d systems (%.1f s)" % (len(rec), args.systems, time.perf_counter() - t0),
+          flush=True)
+    sys_ids = rec[:, proposer.C_SYS]
+    hold = sys_ids % 10 == 9
+    Xtr, ytr = proposer.dataset(rec[~hold])
+    Xte, yte = proposer.dataset(rec[hold])
+    print("samples: train %d, held out %d" % (len(Xtr), len(Xte)), flush=True)
+    w, (mu, sd), hist = proposer.train(Xtr, ytr, epochs=args.epochs, log_every=max(1, args.epochs // 6))
+    for ep, loss in hist:
+        print("epoch %d loss %.4e" % (ep, loss), flush=True)
+    pred = proposer.predict(w, Xte)
+    mse = float(np.mean((pred - yte) ** 2))
+    base = float(np.mean(yte ** 2))
+    margin, gain = choose_margin(pred, yte)
+    base_gain = float(np.mean(np.where(0.0 <= yte, 1.0, 0.0)))
+    print("held-out mse %.4e (predict-0 baseline %.4e); margin %.3f dex, step/attempt %.3f vs plain %.3f"
+          % (mse, base, margin, gain, base_gain), flush=True)
+    wm = proposer.with_margin(w, margin)
+    ev = evaluate(wm, args.eval_systems, args.eval_years)
+    print(json.dumps(ev, indent=1), flush=True)
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    report = {"train_systems": args.systems, "train_years": args.years, "samples": int(len(Xtr)),
+              "heldout_mse": mse, "heldout_mse_zero": base, "margin_dex": margin,
+              "heldout_step_per_attempt": gain, "heldout_step_per_attempt_plain": base_gain,
+              "loss_history": hist, "evaluation": ev}
+    np.savez(args.out, weights=wm, mu=mu, sd=sd, report=json.dumps(report))
+    print("saved", args.out)
+
+
+if __name__ == "__main__":
+    main()

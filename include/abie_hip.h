@@ -203,6 +203,14 @@ int abie_ensemble_get_steplog(abie_ensemble *e, double *records, long long *n_re
 /* Load the learned step proposer's weights (FP32, layout of abie_mlp_*). */
 int abie_ensemble_set_proposer(abie_ensemble *e, const float *weights, int n_weights);
 
+/* Engine configurations (threads per group, bodies per thread) and the
+ * rules that map a system size to one: a system of n bodies runs with the
+ * first rule whose max_bodies >= n and whose configuration holds it.  The
+ * rules come from the autotuner (abie_amd/autotune.py). */
+int abie_gr15_num_configs(void);
+int abie_gr15_config(int cfg, int *threads, int *bodies_per_thread);
+int abie_gr15_set_tuning(int n_rules, const int *max_bodies, const int *cfg);
+
 /* ---- learned step proposer: MLP 12 -> 64 -> 64 -> 1, FP32 ---- */
 #define ABIE_MLP_IN 12
 #define ABIE_MLP_HIDDEN 64
