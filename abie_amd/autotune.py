@@ -41,23 +41,31 @@ def set_rules(rules):
     _lib.check(lib.abie_gr15_set_tuning(len(rules), mx, cf), "abie_gr15_set_tuning")
 
 
-def time_config(systems, cfg, n_bodies, years, controller="plain", force_mode="fp64"):
+def time_config(systems, cfg, n_bodies, years, controller="plain", force_mode="fp64", min_wall=0.4):
+    """Force evaluations per second of one configuration; the timed run is
+    extended until it lasts at least ``min_wall`` seconds."""
     set_rules([(n_bodies, cfg)])
     ens = engine.Ensemble(systems, force_mode=force_mode, controller=controller)
-    ens.run(years * 0.1)  # warm: first launch, code object load
+    t = years * 0.1
+    ens.run(t)  # warm: first launch, code object load
+    ev0 = int(ens.counters()[:, 0].sum())
+    span = years
     t0 = time.perf_counter()
-    ens.run(years)
-    wall = time.perf_counter() - t0
-    cnt = ens.counters()
+    while True:
+        t += span
+        ens.run(t)
+        wall = time.perf_counter() - t0
+        if wall >= min_wall:
+            break
+        span *= 2.0
+    evals = int(ens.counters()[:, 0].sum()) - ev0
     ens.close()
-    evals = int(cnt[:, 0].sum())
     return evals / wall, wall
 
 
 def tune(n_copies=256, years=0.01, nm=32, verbose=True):
     cfgs = configs()
     rules, table = [], {}
-    lo = 0
     for b in BUCKETS:
         m = min(nm, b // 2)
         systems = [ics.planetary_system(3000 + k, m, b - m) for k in range(n_copies)]
@@ -80,7 +88,6 @@ def tune(n_copies=256, years=0.01, nm=32, verbose=True):
                 best = (c, rate)
         if best is not None:
             rules.append((b, best[0]))
-        lo = b
     return rules, table
 
 

@@ -9,9 +9,13 @@ namespace abie {
 
 // One independent system: bodies [body0, body0 + n) of the flat arrays, in
 // engine order (massive bodies first, ascending original index, then the
-// rest, ascending).  nm = number of massive bodies (m > 0).
+// rest, ascending).  nm = number of massive bodies (m > 0).  A system split
+// into several units (test-particle slices) has one SysDesc per unit: nt_sys
+// is the whole system's test-particle count, sys_id its index and primary is
+// 1 on the unit whose massive bodies are reported.
 struct SysDesc {
     int body0, n, nm, cfg;
+    int nt_sys, sys_id, primary, pad;
 };
 
 // Per-system control block, device-resident; persists across launches.

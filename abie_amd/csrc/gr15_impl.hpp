@@ -539,7 +539,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 x[4] = (float)pc_used / 12.0f;
                 x[5] = (float)log10(estim + eps);
                 x[6] = (float)log10((double)nm);
-                x[7] = (float)log10((double)(n - nm) + 1.0);
+                x[7] = (float)log10((double)d.nt_sys + 1.0);
                 x[8] = (float)log10(1.0 + (double)rej_before);
                 x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
                 x[10] = (float)ctl->prev_pc / 12.0f;
@@ -563,13 +563,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         prop_pending = (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr);
 
         // step log
-        if (A.log != nullptr && tid == 0) {
+        if (A.log != nullptr && tid == 0 && d.primary) {
             const unsigned long long slot = atomicAdd(A.log_len, 1ull);
             if ((long long)slot < A.log_cap) {
                 double *r = A.log + slot * ABIE_STEPLOG_FIELDS;
-                r[0] = sys; r[1] = t_att; r[2] = h_att; r[3] = estim; r[4] = err; r[5] = dtreq;
+                r[0] = d.sys_id; r[1] = t_att; r[2] = h_att; r[3] = estim; r[4] = err; r[5] = dtreq;
                 r[6] = accepted; r[7] = pc_used; r[8] = rej_before; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
-                r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = n - nm; r[15] = h_next;
+                r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = d.nt_sys; r[15] = h_next;
             }
         }
         if (accepted) {

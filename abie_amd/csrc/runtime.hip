@@ -176,7 +176,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         if (first) e->sys_unit0[s] = u;
         e->unit_sys[u] = s;
         const int n = un.nm + (un.t1 - un.t0);
-        e->desc[u] = SysDesc{(int)g0, n, un.nm, pick_cfg(n, un.nm)};
+        e->desc[u] = SysDesc{(int)g0, n, un.nm, pick_cfg(n, un.nm), (offsets[s + 1] - o) - un.nm, s, first ? 1 : 0, 0};
         for (int p = 0; p < n; ++p) {
             const int ep = p < un.nm ? p : un.t0 + (p - un.nm);  // engine position in the whole system
             const int i = perms[s][ep];
@@ -738,7 +738,7 @@ double calculate_energy(void) {
         }
         h[6 * N + i] = g.m[i];
     }
-    SysDesc d{0, (int)N, 0, 0};
+    SysDesc d{0, (int)N, 0, 0, 0, 0, 1, 0};
     bool ok = hipMemcpy(sd.p, &d, sizeof d, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(buf.p, h.data(), 7 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               launch_energy(sd.p, 1, buf.p, buf.p + N, buf.p + 2 * N, buf.p + 3 * N, buf.p + 4 * N, buf.p + 5 * N,

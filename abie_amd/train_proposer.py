@@ -53,7 +53,10 @@ def choose_margin(pred, y, grid=np.linspace(0.0, 0.3, 31)):
     return float(best_m), float(best)
 
 
-def evaluate(weights, n_sys, years, force_mode="fp64"):
+def evaluate(weights, n_sys, years, force_mode="fp64", max_seconds=120.0):
+    """Massive-only controller against the learned one on the same systems.
+    Each run is bounded by ``max_seconds``; systems that did not reach
+    ``years`` are reported (``unfinished``) and rates use the time reached."""
     out = {}
     systems = ics.ensemble(n_sys, seed=EVAL_SEED)
     for name, ctrl, w in (("massive", "massive", None), ("learned", "learned", weights)):
@@ -62,18 +65,21 @@ def evaluate(weights, n_sys, years, force_mode="fp64"):
             ens.set_proposer(w)
         e0 = ens.energy()
         t0 = time.perf_counter()
-        ens.run(years)
+        done = ens.run(years, max_seconds=max_seconds)
         wall = time.perf_counter() - t0
         _, _, t, _ = ens.state()
         cnt = ens.counters()
         e1 = ens.energy()
         ens.close()
-        out[name] = {"wall_s": wall, "system_yr_per_s": float(t.sum() / wall),
+        late = np.nonzero(t < years)[0]
+        out[name] = {"done": bool(done), "wall_s": wall, "system_yr_per_s": float(t.sum() / wall),
                      "evals_per_yr": float(cnt[:, 0].sum() / t.sum()),
                      "attempts": int(cnt[:, 1].sum()), "accepted": int(cnt[:, 2].sum()),
                      "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1)),
                      "proposals": int(cnt[:, 4].sum()), "proposals_rejected": int(cnt[:, 5].sum()),
-                     "max_dE_E": float(np.max(np.abs((e1 - e0) / e0)))}
+                     "max_dE_E": float(np.nanmax(np.abs((e1 - e0) / e0))),
+                     "unfinished": [(int(k), float(t[k])) for k in late[:20]], "n_unfinished": int(len(late))}
+        print(name, json.dumps(out[name]), flush=True)
     out["evals_ratio"] = out["massive"]["evals_per_yr"] / out["learned"]["evals_per_yr"]
     return out
 
@@ -85,6 +91,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--eval-systems", type=int, default=256)
     ap.add_argument("--eval-years", type=float, default=10.0)
+    ap.add_argument("--eval-seconds", type=float, default=120.0)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "proposer.npz"))
     args = ap.parse_args()
     t0 = time.perf_counter()
@@ -107,13 +114,16 @@ def main():
     print("held-out mse %.4e (predict-0 baseline %.4e); margin %.3f dex, step/attempt %.3f vs plain %.3f"
           % (mse, base, margin, gain, base_gain), flush=True)
     wm = proposer.with_margin(w, margin)
-    ev = evaluate(wm, args.eval_systems, args.eval_years)
-    print(json.dumps(ev, indent=1), flush=True)
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     report = {"train_systems": args.systems, "train_years": args.years, "samples": int(len(Xtr)),
               "heldout_mse": mse, "heldout_mse_zero": base, "margin_dex": margin,
               "heldout_step_per_attempt": gain, "heldout_step_per_attempt_plain": base_gain,
-              "loss_history": hist, "evaluation": ev}
+              "loss_history": hist}
+    np.savez(args.out, weights=wm, mu=mu, sd=sd, report=json.dumps(report))
+    print("saved", args.out, "; evaluating", flush=True)
+    ev = evaluate(wm, args.eval_systems, args.eval_years, max_seconds=args.eval_seconds)
+    print(json.dumps(ev, indent=1), flush=True)
+    report["evaluation"] = ev
     np.savez(args.out, weights=wm, mu=mu, sd=sd, report=json.dumps(report))
     print("saved", args.out)
 

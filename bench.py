@@ -135,6 +135,46 @@ def ensemble_sample(n_sys, years, force_mode, controller, proposer=None):
             "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1))}
 
 
+def mixed_error_report(n_test=100000):
+    """Per-particle relative acceleration error of the mixed mode against FP64
+    on the 1 star + 4 planets system with n_test test particles."""
+    import torch
+    from abie_amd import engine, ics
+    pos, vel, m = ics.config0(n_test=n_test)
+    src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, G)
+    a64 = torch.stack(engine.accel_soa(src, sinks, n_self, "fp64"), 1)
+    amx = torch.stack(engine.accel_soa(src, sinks, n_self, "mixed"), 1)
+    rel = (torch.linalg.norm(amx - a64, dim=1) / torch.linalg.norm(a64, dim=1))[n_self:]
+    return {"n_test": n_test, "max_rel_err": float(rel.max()), "rms_rel_err": float(torch.sqrt((rel ** 2).mean()))}
+
+
+def mixed_long_run(n_test, years, max_seconds=300.0):
+    """GR15 of the 1 star + 4 planets system with n_test test particles under
+    the massive-only controller (test particles sliced over workgroups), FP64
+    vs mixed forces: |dE/E| and the test particles' final separation."""
+    from abie_amd import engine, ics
+    pos, vel, m = ics.config0(n_test=n_test)
+    out = {}
+    fin = {}
+    for fm in ("fp64", "mixed"):
+        ens = engine.Ensemble([(pos, vel, m)], G=G, force_mode=fm, controller="massive")
+        e0 = ens.energy()
+        t0 = time.perf_counter()
+        done = ens.run(years, max_seconds=max_seconds)
+        wall = time.perf_counter() - t0
+        p, v, t, _ = ens.state()
+        e1 = ens.energy()
+        c = ens.counters()
+        ens.close()
+        fin[fm] = (p, t[0])
+        out[fm] = {"done": bool(done), "model_yr": float(t[0]), "wall_s": wall, "dE_E": float(abs((e1[0] - e0[0]) / e0[0])),
+                   "evals": int(c[0, 0]), "system_yr_per_s": float(t[0] / wall)}
+    if fin["fp64"][1] == fin["mixed"][1]:
+        d = np.linalg.norm(fin["mixed"][0][5:] - fin["fp64"][0][5:], axis=1) / np.linalg.norm(fin["fp64"][0][5:], axis=1)
+        out["tp_position_rel_diff"] = {"max": float(d.max()), "median": float(np.median(d))}
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "force_c1a_traffic.json")
     try:
@@ -151,6 +191,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="also run the longer per-config measurements (config 0 full 100 yr, "
+                         "config 2 1,000-yr mixed run, config 3/4 ensemble samples)")
+    ap.add_argument("--proposer", default=os.path.join(ROOT, "abie_amd", "data", "proposer.npz"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the measurement, copy a buffer of this size 5 times "
                          "(reference traffic for hardware byte counters)")
@@ -212,6 +256,20 @@ def main():
         log("mixed/exact: %s %s" % (extras["config1a_mixed"], extras["config1a_exact"]))
         extras["gr15_dropin_config0_10yr"] = gr15_dropin(10.0)
         log("gr15 drop-in: %s" % extras["gr15_dropin_config0_10yr"])
+        extras["mixed_error_config2"] = mixed_error_report()
+        log("mixed error: %s" % extras["mixed_error_config2"])
+        w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
+        ens = {}
+        for fm, ctrl in (("fp64", "plain"), ("mixed", "learned" if w is not None else "massive")):
+            ens["%s/%s" % (fm, ctrl)] = ensemble_sample(4096, 0.05, fm, ctrl, proposer=w if ctrl == "learned" else None)
+            log("ensemble %s/%s: %s" % (fm, ctrl, ens["%s/%s" % (fm, ctrl)]))
+        k = list(ens)
+        ens["chosen_over_plain_system_yr_per_s"] = ens[k[1]]["system_yr_per_s"] / ens[k[0]]["system_yr_per_s"]
+        extras["ensemble_config3_sample"] = ens
+    if args.configs:
+        extras["gr15_dropin_config0_100yr"] = gr15_dropin(100.0)
+        extras["mixed_run_config2"] = mixed_long_run(100000, 1000.0)
+        log("config 2 run: %s" % extras["mixed_run_config2"])
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 8
         a = torch.ones(nel, dtype=torch.float64, device="cuda")

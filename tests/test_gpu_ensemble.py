@@ -35,8 +35,10 @@ def test_sliced_system_matches_unsliced_subsets():
 
 
 def test_plain_controller_rejects_oversized_system():
+    """The plain controller needs the whole system in one workgroup (at most
+    2,560 bodies); a larger one is refused with a clear error."""
     from abie_amd import engine, ics, _lib
-    pos, vel, m = ics.planetary_system(3, 5, 1500)
+    pos, vel, m = ics.planetary_system(3, 5, 3000)
     with pytest.raises(_lib.AbieHipError):
         engine.Ensemble([(pos, vel, m)], controller="plain")
 
