@@ -38,6 +38,11 @@ constexpr double kR[8][8] = ABIE_RADAU_RS;
 constexpr double kC[8][8] = ABIE_RADAU_CS;
 // integrator_gauss_radau15.h:17,58-63
 constexpr double kEpsb = 1.e-4, kFac = 0.25, kTolpc = 1.e-16, kHmin = 1.e-13;
+// The learned proposer's output (log10 of the proposal / dtreq) is clamped to
+// +-kPropClamp: the logged targets have a spread of ~0.09 dex, and a wider
+// range lets an out-of-distribution output hold h well below dtreq step after
+// step, which shrinks h without bound (err stays ~10^-clamp, dtreq ~ 10 h).
+constexpr double kPropClamp = 0.3;
 
 // Every FP64 constant of the step is read from this table through a pointer
 // the compiler cannot see through (launder()), so each use is a scalar load
@@ -532,12 +537,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             if (tid < 64) {
                 float x[ABIE_MLP_IN];
                 const double eps = 1e-300;
-                x[0] = (float)log10(h_att);
+                x[0] = (float)log10(ctl->prev_dtreq > 0 ? h_att / ctl->prev_dtreq : 1.0);
                 x[1] = (float)log10(err + eps);
                 x[2] = (float)log10(ctl->prev_err + eps);
                 x[3] = (float)log10(h_att / (ctl->prev_h > 0 ? ctl->prev_h : h_att));
                 x[4] = (float)pc_used / 12.0f;
-                x[5] = (float)log10(estim + eps);
+                x[5] = 0.0f;  // reserved
                 x[6] = (float)log10((double)nm);
                 x[7] = (float)log10((double)d.nt_sys + 1.0);
                 x[8] = (float)log10(1.0 + (double)rej_before);
@@ -548,7 +553,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 if (tid == 0) flag[0] = (double)yl;
             }
             __syncthreads();
-            double ratio = exp10(fmin(fmax(flag[0], -1.0), 1.0));
+            double ratio = exp10(fmin(fmax(flag[0], -kPropClamp), kPropClamp));
             double hp = dtreq * ratio;
             // keep the reference's growth / shrink bounds
             hp = fmin(hp, h_att / kFac);

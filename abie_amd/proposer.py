@@ -12,10 +12,13 @@ h_{i+1} = dtreq_i * 10^(y_hat - margin) instead; GR15's own error test still
 accepts or rejects it, and a rejection falls back to the plain rule.
 
 Features (ABIE_MLP_IN = 12, computed identically on the device, see
-gr15_impl.hpp): log10 h_i, log10 err_i, log10 err_{i-1}, log10(h_i/h_{i-1}),
-pc_i/12, log10 estim_i, log10 n_massive, log10(n_test+1),
-log10(1 + rejections before step i), log10(dtreq_i/dtreq_{i-1}),
-pc_{i-1}/12, 1.
+gr15_impl.hpp): log10(h_i/dtreq_{i-1}), log10 err_i, log10 err_{i-1},
+log10(h_i/h_{i-1}), pc_i/12, 0 (reserved), log10 n_massive,
+log10(n_test+1), log10(1 + rejections before step i),
+log10(dtreq_i/dtreq_{i-1}), pc_{i-1}/12, 1.  Every feature is invariant
+under a change of time scale: an absolute step size is out of distribution
+near close encounters, where a network that saw it could keep shrinking h.
+The device clamps the output to +-0.3 dex.
 
 Training runs on the GPU with the project's HIP kernels (abie_mlp_loss_grad,
 abie_mlp_adam); standardisation is folded into the first layer afterwards so
@@ -36,12 +39,12 @@ def features(rec):
     h, err, est, dtreq = r[:, C_H], r[:, C_ERR], r[:, C_EST], r[:, C_DTREQ]
     ph, perr, pdt = r[:, C_PH], r[:, C_PERR], r[:, C_PDT]
     x = np.empty((len(r), _lib.MLP_IN), dtype=np.float64)
-    x[:, 0] = np.log10(h)
+    x[:, 0] = np.log10(np.where(pdt > 0, h / np.where(pdt > 0, pdt, 1.0), 1.0))
     x[:, 1] = np.log10(err + EPS)
     x[:, 2] = np.log10(perr + EPS)
     x[:, 3] = np.log10(h / np.where(ph > 0, ph, h))
     x[:, 4] = r[:, C_PC] / 12.0
-    x[:, 5] = np.log10(est + EPS)
+    x[:, 5] = 0.0
     x[:, 6] = np.log10(r[:, C_NM])
     x[:, 7] = np.log10(r[:, C_NT] + 1.0)
     x[:, 8] = np.log10(1.0 + r[:, C_REJ])

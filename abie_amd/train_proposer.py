@@ -40,12 +40,16 @@ def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None):
     return rec
 
 
+CLAMP_DEX = 0.3  # the device's clamp of the proposer output (gr15_impl.hpp)
+
+
 def choose_margin(pred, y, grid=np.linspace(0.0, 0.3, 31)):
     """Margin that maximises expected step length per attempt: a proposal
-    dtreq*10^(p-m) is accepted when p - m <= y (the next dtreq ratio)."""
+    dtreq*10^(p-m) (clamped as on the device) is accepted when p - m <= y
+    (the next dtreq ratio)."""
     best, best_m = -1.0, 0.0
     for m in grid:
-        prop = pred - m
+        prop = np.clip(pred - m, -CLAMP_DEX, CLAMP_DEX)
         ok = prop <= y
         gain = np.mean(np.where(ok, 10.0 ** prop, 0.0))
         if gain > best:
@@ -102,7 +106,9 @@ def main():
     hold = sys_ids % 10 == 9
     Xtr, ytr = proposer.dataset(rec[~hold])
     Xte, yte = proposer.dataset(rec[hold])
-    print("samples: train %d, held out %d" % (len(Xtr), len(Xte)), flush=True)
+    q = np.quantile(ytr, [0.001, 0.01, 0.5, 0.99, 0.999])
+    print("samples: train %d, held out %d; target quantiles (0.1%%, 1%%, 50%%, 99%%, 99.9%%) %s"
+          % (len(Xtr), len(Xte), np.array2string(q, precision=3)), flush=True)
     w, (mu, sd), hist = proposer.train(Xtr, ytr, epochs=args.epochs, log_every=max(1, args.epochs // 6))
     for ep, loss in hist:
         print("epoch %d loss %.4e" % (ep, loss), flush=True)
@@ -118,7 +124,7 @@ def main():
     report = {"train_systems": args.systems, "train_years": args.years, "samples": int(len(Xtr)),
               "heldout_mse": mse, "heldout_mse_zero": base, "margin_dex": margin,
               "heldout_step_per_attempt": gain, "heldout_step_per_attempt_plain": base_gain,
-              "loss_history": hist}
+              "target_quantiles": q.tolist(), "loss_history": hist}
     np.savez(args.out, weights=wm, mu=mu, sd=sd, report=json.dumps(report))
     print("saved", args.out, "; evaluating", flush=True)
     ev = evaluate(wm, args.eval_systems, args.eval_years, max_seconds=args.eval_seconds)
