@@ -3,10 +3,12 @@
     python -m abie_amd.autotune [--out abie_amd/data/gr15_tuning.json]
 
 For each size bucket (upper bound on bodies per system) it builds an ensemble
-of copies of a representative planetary system (32 massive bodies, the rest
-test particles, sized at the bucket's upper bound), runs it once with every
-engine configuration (threads per group x bodies per thread) that holds it,
-and keeps the configuration with the most force evaluations per second.  The
+of identical copies of one planetary system (32 massive bodies, the rest test
+particles, sized at the bucket's upper bound; identical copies do identical
+work, so the wall time measures the engine rather than the slowest system),
+runs it with every engine configuration (threads per group x bodies per
+thread) that holds it, and keeps the configuration with the most force
+evaluations per second.  The
 result is a rules table {max_bodies -> configuration} that the library uses
 through abie_gr15_set_tuning (applied automatically when the table exists).
 """
@@ -63,12 +65,12 @@ def time_config(systems, cfg, n_bodies, years, controller="plain", force_mode="f
     return evals / wall, wall
 
 
-def tune(n_copies=256, years=0.01, nm=32, verbose=True):
+def tune(n_copies=1024, years=0.01, nm=32, verbose=True):
     cfgs = configs()
     rules, table = [], {}
     for b in BUCKETS:
         m = min(nm, b // 2)
-        systems = [ics.planetary_system(3000 + k, m, b - m) for k in range(n_copies)]
+        systems = [ics.planetary_system(3000 + b, m, b - m)] * n_copies
         best = None
         table[b] = {}
         for c, (nt, bpt) in enumerate(cfgs):
@@ -105,7 +107,7 @@ def load_rules(path=DEFAULT_OUT):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_OUT)
-    ap.add_argument("--copies", type=int, default=256)
+    ap.add_argument("--copies", type=int, default=1024)
     ap.add_argument("--years", type=float, default=0.01)
     args = ap.parse_args()
     rules, table = tune(args.copies, args.years)

@@ -39,10 +39,13 @@ constexpr double kC[8][8] = ABIE_RADAU_CS;
 // integrator_gauss_radau15.h:17,58-63
 constexpr double kEpsb = 1.e-4, kFac = 0.25, kTolpc = 1.e-16, kHmin = 1.e-13;
 // The learned proposer's output (log10 of the proposal / dtreq) is clamped to
-// +-kPropClamp: the logged targets have a spread of ~0.09 dex, and a wider
-// range lets an out-of-distribution output hold h well below dtreq step after
-// step, which shrinks h without bound (err stays ~10^-clamp, dtreq ~ 10 h).
-constexpr double kPropClamp = 0.3;
+// [kPropLo, kPropHi].  kPropLo: the logged targets have a spread of ~0.09 dex,
+// and a wider range lets an out-of-distribution output hold h well below
+// dtreq step after step, which shrinks h without bound (err stays
+// ~10^kPropLo, dtreq ~ 10 h).  kPropHi = 0: a proposal never exceeds the
+// plain controller's dtreq, so each step is at most as long as the plain
+// rule's and the proposer only removes rejected attempts.
+constexpr double kPropLo = -0.3, kPropHi = 0.0;
 
 // Every FP64 constant of the step is read from this table through a pointer
 // the compiler cannot see through (launder()), so each use is a scalar load
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 if (tid == 0) flag[0] = (double)yl;
             }
             __syncthreads();
-            double ratio = exp10(fmin(fmax(flag[0], -kPropClamp), kPropClamp));
+            double ratio = exp10(fmin(fmax(flag[0], kPropLo), kPropHi));
             double hp = dtreq * ratio;
             // keep the reference's growth / shrink bounds
             hp = fmin(hp, h_att / kFac);

@@ -18,7 +18,8 @@ log10(n_test+1), log10(1 + rejections before step i),
 log10(dtreq_i/dtreq_{i-1}), pc_{i-1}/12, 1.  Every feature is invariant
 under a change of time scale: an absolute step size is out of distribution
 near close encounters, where a network that saw it could keep shrinking h.
-The device clamps the output to +-0.3 dex.
+The device clamps the output to [-0.3, 0] dex: a proposal is never longer
+than the plain controller's dtreq.
 
 Training runs on the GPU with the project's HIP kernels (abie_mlp_loss_grad,
 abie_mlp_adam); standardisation is folded into the first layer afterwards so

@@ -40,16 +40,16 @@ def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None):
     return rec
 
 
-CLAMP_DEX = 0.3  # the device's clamp of the proposer output (gr15_impl.hpp)
+CLAMP_LO, CLAMP_HI = -0.3, 0.0  # the device's clamp of the proposer output (gr15_impl.hpp)
 
 
-def choose_margin(pred, y, grid=np.linspace(0.0, 0.3, 31)):
+def choose_margin(pred, y, grid=np.linspace(-0.2, 0.3, 51)):
     """Margin that maximises expected step length per attempt: a proposal
     dtreq*10^(p-m) (clamped as on the device) is accepted when p - m <= y
     (the next dtreq ratio)."""
     best, best_m = -1.0, 0.0
     for m in grid:
-        prop = np.clip(pred - m, -CLAMP_DEX, CLAMP_DEX)
+        prop = np.clip(pred - m, CLAMP_LO, CLAMP_HI)
         ok = prop <= y
         gain = np.mean(np.where(ok, 10.0 ** prop, 0.0))
         if gain > best:

@@ -11,9 +11,14 @@ events on the launch stream (roofline.achieved), its HBM traffic from the
 committed rocprofv3 counter summary (profiles/force_c1a_traffic.json).
 
 Extras (reported beside the headline, not part of it): the 65,536-body
-Plummer force case, the mixed-precision force mode, the drop-in GR15 on the
-1 star + 4 planets + 1,000 test particle system, and a sample of the 4,096
-system ensemble in plain FP64 and in the chosen form.
+Plummer force case, the mixed-precision and exact force modes, the drop-in
+GR15 on the 1 star + 4 planets + 1,000 test particle system (GPU, and the
+reference's own C GR15 on one host core), the mixed-precision force error on
+that system with 100,000 test particles, and the 4,096-system ensemble
+(0.2 yr per system) under the plain FP64 GR15, the massive-only controller
+and the learned step proposer (FP64 and mixed test forces).  --configs adds
+the 100-yr drop-in run, the 1,000-yr mixed run with 100,000 test particles
+and a 1-yr ensemble pass.
 
 The CPU baseline times the reference's own C force loop (oracle/_ref, built
 from the reference sources) -- or the oracle's restatement when that is
@@ -114,9 +119,10 @@ def gr15_dropin(years, force_mode="fp64"):
     return {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "dE_E": abs((e1 - e0) / e0)}
 
 
-def ensemble_sample(n_sys, years, force_mode, controller, proposer=None):
-    from abie_amd import engine, ics
-    systems = ics.ensemble(n_sys)
+def ensemble_sample(systems, years, force_mode, controller, proposer=None):
+    """One pass of the batched GR15 engine over an ensemble (state resident on
+    the device), bounded to 120 s; rates use the time each system reached."""
+    from abie_amd import engine
     ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller=controller)
     if proposer is not None:
         ens.set_proposer(proposer)
@@ -129,10 +135,60 @@ def ensemble_sample(n_sys, years, force_mode, controller, proposer=None):
     cnt = ens.counters()
     ens.close()
     de = np.abs((e1 - e0) / e0)
-    return {"n_systems": n_sys, "model_yr_each": years, "done": bool(done), "wall_s": wall,
-            "system_yr_per_s": float(t.sum() / wall), "max_dE_E": float(de.max()),
-            "evals_per_yr": float(cnt[:, 0].sum() / t.sum()),
-            "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1))}
+    return {"n_systems": len(systems), "model_yr_each": years, "done": bool(done), "wall_s": wall,
+            "system_yr_per_s": float(t.sum() / wall), "evals_per_system_yr": float(cnt[:, 0].sum() / t.sum()),
+            "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1)),
+            "dE_E_max": float(de.max()), "dE_E_p99": float(np.quantile(de, 0.99)),
+            "dE_E_median": float(np.median(de)), "systems_above_1e-10": int((de > 1e-10).sum())}
+
+
+def ensemble_report(n_sys, years, proposer):
+    """Plain FP64 GR15 against the massive-only controller, the learned
+    proposer, and the learned proposer with mixed-precision test forces, on
+    the same ensemble."""
+    from abie_amd import ics
+    systems = ics.ensemble(n_sys)
+    runs = [("fp64/plain", "fp64", "plain"), ("fp64/massive", "fp64", "massive")]
+    if proposer is not None:
+        runs += [("fp64/learned", "fp64", "learned"), ("mixed/learned", "mixed", "learned")]
+    else:
+        runs += [("mixed/massive", "mixed", "massive")]
+    out = {}
+    for name, fm, ctrl in runs:
+        out[name] = ensemble_sample(systems, years, fm, ctrl, proposer if ctrl == "learned" else None)
+        log("ensemble %s: %s" % (name, out[name]))
+    base = out["fp64/plain"]["system_yr_per_s"]
+    out["speedup_vs_plain"] = {k: v["system_yr_per_s"] / base for k, v in out.items() if k != "fp64/plain"}
+    return out
+
+
+def cpu_gr15_config0(years):
+    """The reference's own GR15 (oracle/_ref, device -1) on the 1 star + 4
+    planets + 1,000 test particle system, one host core."""
+    import ctypes
+    from abie_amd import ics
+    ref = os.path.join(ROOT, "oracle", "_ref", "libabie_ref.so")
+    if not os.path.exists(ref):
+        return None
+    L = ctypes.CDLL(ref)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.initialize_code.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_size_t, ctypes.c_size_t,
+                                  ctypes.c_size_t, ctypes.c_int]
+    L.set_state.argtypes = [dp, dp, dp, dp, ctypes.c_size_t, ctypes.c_double, ctypes.c_double]
+    L.integrator_gr.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.integrator_gr.restype = ctypes.c_int
+    L.get_model_time.restype = ctypes.c_double
+    pos, vel, m = ics.config0()
+    p, v, mm = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy()
+    r = np.zeros_like(mm)
+    L.initialize_code(G, 0.0, len(m), 1, 1, -1)
+    L.set_state(p.ctypes.data_as(dp), v.ctypes.data_as(dp), mm.ctypes.data_as(dp), r.ctypes.data_as(dp), len(m), G, 0.0)
+    t0 = time.perf_counter()
+    L.integrator_gr(0.0, years, 1.0)
+    wall = time.perf_counter() - t0
+    t = L.get_model_time()
+    L.finalize_code()
+    return {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "cores": 1, "kind": "reference"}
 
 
 def mixed_error_report(n_test=100000):
@@ -204,6 +260,11 @@ def main():
               "(the ensemble shards by system with no communication).", file=sys.stderr)
         sys.exit(2)
 
+    # Everything the libraries print (the reference's C code, device printf)
+    # goes to stderr; stdout carries only the JSON line.
+    json_out = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
+
     import torch
     from abie_amd import _lib, ics
     _lib.load()
@@ -258,18 +319,17 @@ def main():
         log("gr15 drop-in: %s" % extras["gr15_dropin_config0_10yr"])
         extras["mixed_error_config2"] = mixed_error_report()
         log("mixed error: %s" % extras["mixed_error_config2"])
+        extras["gr15_config0_cpu_reference_1yr"] = cpu_gr15_config0(1.0) if not args.no_cpu else None
+        log("gr15 cpu reference: %s" % extras["gr15_config0_cpu_reference_1yr"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
-        ens = {}
-        for fm, ctrl in (("fp64", "plain"), ("mixed", "learned" if w is not None else "massive")):
-            ens["%s/%s" % (fm, ctrl)] = ensemble_sample(4096, 0.05, fm, ctrl, proposer=w if ctrl == "learned" else None)
-            log("ensemble %s/%s: %s" % (fm, ctrl, ens["%s/%s" % (fm, ctrl)]))
-        k = list(ens)
-        ens["chosen_over_plain_system_yr_per_s"] = ens[k[1]]["system_yr_per_s"] / ens[k[0]]["system_yr_per_s"]
-        extras["ensemble_config3_sample"] = ens
+        extras["ensemble_config3_4096x0.2yr"] = ensemble_report(4096, 0.2, w)
     if args.configs:
         extras["gr15_dropin_config0_100yr"] = gr15_dropin(100.0)
+        log("config 0, 100 yr: %s" % extras["gr15_dropin_config0_100yr"])
         extras["mixed_run_config2"] = mixed_long_run(100000, 1000.0)
         log("config 2 run: %s" % extras["mixed_run_config2"])
+        w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
+        extras["ensemble_config3_4096x1yr"] = ensemble_report(4096, 1.0, w)
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 8
         a = torch.ones(nel, dtype=torch.float64, device="cuda")
@@ -284,7 +344,11 @@ def main():
         log("cpu: %s" % out["cpu_baseline"])
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out))
+    import ctypes
+    sys.stdout.flush()
+    ctypes.CDLL(None).fflush(None)
+    json_out.write(json.dumps(out) + "\n")
+    json_out.flush()
 
 
 if __name__ == "__main__":
