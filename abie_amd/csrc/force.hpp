@@ -16,8 +16,9 @@ struct AccelArgs {
 };
 
 struct AccelPlan {
-    int spt = 4;    // sinks per thread
-    int parts = 1;  // source-range split (partials summed in order)
+    int spt = 4;          // sinks per thread
+    int parts = 1;        // source-range split (partials summed in order)
+    int scalar_src = 0;   // 1: sources through scalar loads (accel_scalar_kernel)
 };
 
 AccelPlan plan_accel(int n_src, int n_sink, int mode, int n_cu);

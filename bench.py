@@ -247,6 +247,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the 4,096-system ensemble extras (for kernel traces of the GR15 engine)")
     ap.add_argument("--configs", action="store_true",
                     help="also run the longer per-config measurements (config 0 full 100 yr, "
                          "config 2 1,000-yr mixed run, config 3/4 ensemble samples)")
@@ -269,6 +271,13 @@ def main():
     from abie_amd import _lib, ics
     _lib.load()
     torch.cuda.init()
+
+    if args.ensemble_only:
+        w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
+        rep = ensemble_report(4096, 0.2, w)
+        json_out.write(json.dumps({"ensemble_config3_4096x0.2yr": rep}) + "\n")
+        json_out.flush()
+        return
 
     pos, vel, m = ics.config1a()
     n_m, n_t = int((m > 0).sum()), int((m == 0).sum())
@@ -297,7 +306,7 @@ def main():
                    "n_massive": n_m, "n_test": n_t, "interactions_per_step": n_int, "mode": "fp64"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
-                     "kernel": "accel_kernel<4, FP64>", "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "kernel": "accel_scalar_kernel<1, FP64>", "flop_per_interaction": FLOP_PER_INTERACTION,
                      "mean_launch_us": ms_event * 1e3},
     }
     extras = {}
