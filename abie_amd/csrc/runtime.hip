@@ -872,25 +872,47 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz, con
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz, const double *gm, int n_src,
                             const double *px, const double *py, const double *pz, int n_sink, int n_self,
                             double *ax, double *ay, double *az, int mode, void *stream, int reps,
-                            double *ms_per_launch) {
+                            double *ms_per_launch, double *ms_each) {
     if (reps <= 0 || !ms_per_launch) return fail(ABIE_HIP_EINVAL, "time_accel_soa: bad arguments");
     hipStream_t st = (hipStream_t)stream;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    std::vector<hipEvent_t> ev(ms_each ? 2 * reps : 2);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    auto destroy = [&]() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    };
     int rc = 0;
-    HIP_TRY(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) {
+    HIP_TRY(hipEventRecord(ev[0], st));
+    for (int r = 0; r < reps && !rc; ++r)
         rc = abie_hip_accel_soa(sx, sy, sz, gm, n_src, px, py, pz, n_sink, n_self, ax, ay, az, mode, stream);
-        if (rc) return rc;
+    if (rc) {
+        destroy();
+        return rc;
     }
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev[1], st));
+    HIP_TRY(hipEventSynchronize(ev[1]));
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
     *ms_per_launch = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (ms_each) {
+        for (int r = 0; r < reps && !rc; ++r) {
+            HIP_TRY(hipEventRecord(ev[2 * r], st));
+            rc = abie_hip_accel_soa(sx, sy, sz, gm, n_src, px, py, pz, n_sink, n_self, ax, ay, az, mode, stream);
+            HIP_TRY(hipEventRecord(ev[2 * r + 1], st));
+        }
+        if (rc) {
+            destroy();
+            return rc;
+        }
+        HIP_TRY(hipEventSynchronize(ev[2 * reps - 1]));
+        double tot = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            float m1 = 0;
+            HIP_TRY(hipEventElapsedTime(&m1, ev[2 * r], ev[2 * r + 1]));
+            tot += m1;
+        }
+        *ms_each = tot / reps;
+    }
+    destroy();
     return 0;
 }
 

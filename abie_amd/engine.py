@@ -58,19 +58,22 @@ def accel_soa(src, sinks, n_self, mode="fp64", out=None, stream=None):
     return out
 
 
-def time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=100, stream=None):
+def time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=100, stream=None, each=False):
     """Mean milliseconds per launch of ``reps`` back-to-back launches, timed
-    with HIP events recorded on the launch stream."""
+    with HIP events recorded on the launch stream.  each=True also returns
+    the mean duration of a launch bracketed by its own event pair."""
     sx, sy, sz, gm = src
     px, py, pz = sinks
     ax, ay, az = out
     ms = ctypes.c_double(0.0)
+    ms1 = ctypes.c_double(0.0)
     lib = _lib.load()
     _lib.check(lib.abie_hip_time_accel_soa(_ptr(sx), _ptr(sy), _ptr(sz), _ptr(gm), sx.numel(), _ptr(px), _ptr(py),
                                            _ptr(pz), px.numel(), int(n_self), _ptr(ax), _ptr(ay), _ptr(az),
-                                           _mode(mode), _stream(stream), int(reps), ctypes.byref(ms)),
+                                           _mode(mode), _stream(stream), int(reps), ctypes.byref(ms),
+                                           ctypes.byref(ms1) if each else None),
                "abie_hip_time_accel_soa")
-    return ms.value
+    return (ms.value, ms1.value) if each else ms.value
 
 
 def soa_from_bodies(pos, m, G, device="cuda"):

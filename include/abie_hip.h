@@ -152,12 +152,16 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz,
                        int mode, void *stream);
 
 /* Times `reps` back-to-back launches of abie_hip_accel_soa on `stream` with
- * HIP events recorded on that stream; *ms_per_launch receives the mean. */
+ * HIP events recorded on that stream; *ms_per_launch receives the mean.
+ * When ms_each is not NULL a second pass brackets each launch with its own
+ * event pair and *ms_each receives the mean single-launch duration (the
+ * quantity a per-dispatch kernel trace reports). */
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz,
                             const double *gm, int n_src,
                             const double *px, const double *py, const double *pz,
                             int n_sink, int n_self, double *ax, double *ay, double *az,
-                            int mode, void *stream, int reps, double *ms_per_launch);
+                            int mode, void *stream, int reps, double *ms_per_launch,
+                            double *ms_each);
 
 /* ---- batched GR15 ensemble: many independent systems, one launch ---- */
 typedef struct abie_ensemble abie_ensemble;
