@@ -329,18 +329,22 @@ struct PropFeat {
     float f[ABIE_MLP_IN];
 };
 
-__device__ float mlp_eval(const float *__restrict__ w, const float *x, float *lds_h) {
+__device__ __forceinline__ float mlp_eval(const float *__restrict__ w, const float *x, float *lds_h) {
     // 12 -> 64 (ReLU) -> 64 (ReLU) -> 1; one wave, lane = hidden unit
     const int lane = threadIdx.x & 63;
     const float *W1 = w, *b1 = W1 + ABIE_MLP_IN * ABIE_MLP_HIDDEN;
     const float *W2 = b1 + ABIE_MLP_HIDDEN, *b2 = W2 + ABIE_MLP_HIDDEN * ABIE_MLP_HIDDEN;
     const float *w3 = b2 + ABIE_MLP_HIDDEN, *b3 = w3 + ABIE_MLP_HIDDEN;
+    // short unrolls: the GR15 state of the caller is live across this call
+    // and a fully unrolled 64-term loop would spill it
     float h = b1[lane];
+#pragma unroll 4
     for (int i = 0; i < ABIE_MLP_IN; ++i) h = fmaf(x[i], W1[i * ABIE_MLP_HIDDEN + lane], h);
     h = fmaxf(h, 0.0f);
     lds_h[lane] = h;
     __builtin_amdgcn_wave_barrier();
     float h2 = b2[lane];
+#pragma unroll 4
     for (int i = 0; i < ABIE_MLP_HIDDEN; ++i) h2 = fmaf(lds_h[i], W2[i * ABIE_MLP_HIDDEN + lane], h2);
     h2 = fmaxf(h2, 0.0f);
     float o = h2 * w3[lane];

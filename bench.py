@@ -168,6 +168,26 @@ def ensemble_report(n_sys, years, proposer):
     return out
 
 
+def host_aos_case(pos, m, reps=5):
+    """The reference-ABI entry point calculate_accelerations on host AoS
+    arrays (PCIe copies in and out included), device 0."""
+    from abie_amd.clibabie import CLibABIE
+    c = CLibABIE()
+    c.lib.finalize_code()
+    n = len(m)
+    c.initialize_code(G, 0.0, n, 1, 1, 0.0, deviceID=0)
+    x = np.ascontiguousarray(pos.reshape(-1))
+    v = np.zeros_like(x)
+    acc = np.zeros_like(x)
+    c.calculate_accelerations(x, v, n, G, m, acc=acc)  # warm: allocation, first copies
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c.calculate_accelerations(x, v, n, G, m, acc=acc)
+    dt = (time.perf_counter() - t0) / reps
+    c.lib.finalize_code()
+    return dt
+
+
 def cpu_gr15_config0(years):
     """The reference's own GR15 (oracle/_ref, device -1) on the 1 star + 4
     planets + 1,000 test particle system, one host core."""
@@ -334,6 +354,11 @@ def main():
         w, ms, _ = force_case(pos, m, G, "exact", reps, 3)
         extras["config1a_exact"] = {"interactions_per_s": n_int * reps / w, "mean_launch_us": ms * 1e3}
         log("mixed/exact: %s %s" % (extras["config1a_mixed"], extras["config1a_exact"]))
+        dt = host_aos_case(pos, m)
+        extras["config1a_host_aos_pcie"] = {"interactions_per_s": n_int / dt, "ms_per_call": dt * 1e3,
+                                            "note": "calculate_accelerations on host AoS arrays: pageable "
+                                                    "copies over PCIe in and out, reorder kernels, force kernel"}
+        log("host AoS: %s" % extras["config1a_host_aos_pcie"])
         extras["gr15_dropin_config0_10yr"] = gr15_dropin(10.0)
         log("gr15 drop-in: %s" % extras["gr15_dropin_config0_10yr"])
         extras["mixed_error_config2"] = mixed_error_report()
