@@ -1,0 +1,82 @@
+"""The remaining reference entry points of the path, called directly through
+the C ABI: integrator_gauss_radau15 on caller arrays
+(src/integrator_gauss_radau15.c:221), ode_n_body_second_order_gpu with the
+gpu_init / gpu_finalize lifetime (src/gpuforce.cu:88-170), and the error
+codes that replace the reference's exit() calls."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 4 * np.pi ** 2
+
+
+def same_bits(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    return bool(np.all(a.view(np.int64) == b.view(np.int64)))
+
+
+@pytest.fixture()
+def lib():
+    from abie_amd.clibabie import CLibABIE
+    c = CLibABIE()
+    c.lib.finalize_code()
+    yield c
+    c.lib.abie_hip_set_force_mode(0)
+    c.lib.finalize_code()
+
+
+def test_integrator_gauss_radau15_on_caller_arrays(lib, golden):
+    """One 1-yr chunk of the reference's golden run, by the direct call:
+    positions, velocities and model time bit for bit (exact forces)."""
+    d = golden("gr15")
+    m = d["small_m"].copy()
+    pos = d["small_pos0"].reshape(-1).copy()
+    vel = d["small_vel0"].reshape(-1).copy()
+    lib.initialize_code(G, 0.0, len(m), 1, 1, 0.0, deviceID=0)
+    lib.lib.abie_hip_set_force_mode(1)
+    ret = lib.integrator_gauss_radau15(pos, vel, m, len(m), G, 0.0, 1.0, 1.0)
+    assert ret == d["small_ret"][0]
+    assert lib.get_model_time() == d["small_t"][0]
+    assert same_bits(pos, d["small_pos"][0]) and same_bits(vel, d["small_vel"][0])
+
+
+def test_ode_n_body_second_order_gpu_lifetime(lib, golden):
+    """gpu_init brings the device up without initialize_code; the force
+    entry point then matches the reference bit for bit in exact mode;
+    after gpu_finalize it reports an error instead of exiting."""
+    from abie_amd import _lib
+    d = golden("forces")
+    pos, m = d["c0_pos"].reshape(-1), np.ascontiguousarray(d["c0_m"])
+    Gc = float(d["G"])
+    lib.lib.gpu_init(len(m), 0)
+    lib.lib.abie_hip_set_force_mode(1)
+    acc = np.zeros(pos.size)
+    call = lambda: ctypes_call(lib, pos, m, Gc, acc)
+    rc = call()
+    assert rc == 0
+    assert same_bits(acc.reshape(-1, 3), d["c0_acc"])
+    lib.lib.gpu_finalize()
+    rc = call()
+    assert rc != 0 and "initialize_code" in _lib.load().abie_hip_last_error().decode()
+
+
+def ctypes_call(lib, pos, m, Gc, acc):
+    import ctypes
+    from abie_amd import _lib
+    return ctypes.c_long(lib.lib.ode_n_body_second_order_gpu(
+        np.ascontiguousarray(pos).ctypes.data_as(_lib.c_dp), len(m), Gc, m.ctypes.data_as(_lib.c_dp),
+        np.zeros(len(m)).ctypes.data_as(_lib.c_dp), acc.ctypes.data_as(_lib.c_dp))).value
+
+
+def test_invalid_device_and_unsupported_calls(lib):
+    """Where the reference falls back to the CPU or exits, the library
+    returns an error code and names it."""
+    from abie_amd import _lib
+    L = _lib.load()
+    assert L.initialize_code(G, 0.0, 8, 1, 1, -1) == -2  # ABIE_HIP_ENODEV: no CPU fallback
+    assert L.initialize_code(G, 0.0, 8, 1, 1, 99) == -2
+    L.finalize_code()
+    assert L.initialize_code(G, 0.0, 8, 1, 1, 0) == 0
+    assert L.integrator_rk(0.0, 1.0, 0.1) < 0 and L.integrator_wh(0.0, 1.0, 0.1) < 0
+    assert b"outside" in L.abie_hip_last_error()
