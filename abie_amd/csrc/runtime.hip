@@ -262,6 +262,10 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
 static int ens_rings(abie_ensemble *e, int max_ce, int max_coll) {
     e->max_ce = max_ce;
     e->max_coll = max_coll;
+    // Without rings nothing observable depends on the close-encounter /
+    // collision scan (the event counters only stop a system through the
+    // rings' limits), so the O(n^2) pass after each accepted step is skipped.
+    e->events = (max_ce > 0 || max_coll > 0) ? 1 : 0;
     if (max_ce > 0) {
         HIP_TRY(e->d_ring_ce.reserve((size_t)4 * max_ce * e->n_units));
         HIP_TRY(hipMemset(e->d_ring_ce.p, 0, (size_t)4 * max_ce * e->n_units * sizeof(double)));
