@@ -74,3 +74,18 @@ def test_steplog_records_every_attempt():
         assert len(rs) == cnt[s, 1]
         assert int(rs[:, 6].sum()) == cnt[s, 2]
         assert np.all(np.diff(rs[:, 1]) >= 0)
+
+
+def test_skipped_event_scan_changes_nothing(monkeypatch):
+    """Without event rings the engine skips the close-encounter scan; forcing
+    the scan on gives the same trajectories bit for bit."""
+    from abie_amd import engine, ics
+    systems = [ics.planetary_system(620 + k, 4 + k, 40 * k) for k in range(3)]
+    a = engine.Ensemble(systems, controller="plain")
+    a.run(0.3)
+    monkeypatch.setenv("ABIE_HIP_EVENTS", "1")
+    b = engine.Ensemble(systems, controller="plain")
+    b.run(0.3)
+    pa, va, ta, _ = a.state()
+    pb, vb, tb, _ = b.state()
+    assert np.array_equal(ta, tb) and same_bits(pa, pb) and same_bits(va, vb)
