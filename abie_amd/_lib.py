@@ -74,7 +74,7 @@ SIGNATURES = {
 # constants of include/abie_hip.h
 FORCE_FP64, FORCE_FP64_EXACT, FORCE_MIXED = 0, 1, 2
 CTRL_PLAIN, CTRL_MASSIVE, CTRL_LEARNED = 0, 1, 2
-STEPLOG_FIELDS = 16
+STEPLOG_FIELDS = 18
 MLP_IN, MLP_HIDDEN = 12, 64
 MLP_NPARAM = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
 

@@ -61,6 +61,7 @@ struct GR15Args {
     long long log_cap;
     // learned proposer (FP32 weights) or null
     const float *mlp;
+    const double *mratio;  // per unit: (sum of the other massive masses) / largest mass
     int max_attempts;  // per launch
     int events;        // 0: skip the close-encounter / collision scan
 };

@@ -534,6 +534,43 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         } else {
             rejects += 1;
         }
+        // Shortest encounter timescale among the massive bodies of the new
+        // state, as max |v_ij|^2 / r_ij^2 (a feature of the learned proposer,
+        // and logged): massive bodies put (x, v) in the LDS source area,
+        // free between evaluations, and each sums its row of pairs.
+        double rate2 = 0.0;
+        if (accepted && ((CT == ABIE_CTRL_LEARNED && A.mlp != nullptr) || A.log != nullptr)) {
+            double *L = E.src;
+#pragma unroll
+            for (int k = 0; k < BPT; ++k) {
+                const int b = k * NT + tid;
+                if (b < nm) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        L[6 * b + c] = y0[k][c];
+                        L[6 * b + 3 + c] = dy0[k][c];
+                    }
+                }
+            }
+            __syncthreads();
+            double best = 0.0, unused = 0.0;
+#pragma unroll
+            for (int k = 0; k < BPT; ++k) {
+                const int b = k * NT + tid;
+                if (b >= nm) continue;
+                for (int j = 0; j < nm; ++j) {
+                    if (j == b) continue;
+                    const double dx = y0[k][0] - L[6 * j], dyy = y0[k][1] - L[6 * j + 1], dz = y0[k][2] - L[6 * j + 2];
+                    const double vx = dy0[k][0] - L[6 * j + 3], vy = dy0[k][1] - L[6 * j + 4],
+                                 vz = dy0[k][2] - L[6 * j + 5];
+                    const double r2 = dx * dx + dyy * dyy + dz * dz, v2 = vx * vx + vy * vy + vz * vz;
+                    best = fmax(best, v2 / r2);
+                }
+            }
+            wg_max2<NT>(best, unused, E.red);
+            rate2 = best;
+        }
+
         // step size (integrator_gauss_radau15.c:349-367)
         if (dtreq / h > 1.0 / kFac) h_next_plain = h / kFac;
         else if (dtreq < 1.e-12) h_next_plain = h * kFac;
@@ -551,13 +588,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 x[2] = (float)log10(ctl->prev_err + eps);
                 x[3] = (float)log10(h_att / (ctl->prev_h > 0 ? ctl->prev_h : h_att));
                 x[4] = (float)pc_used / 12.0f;
-                x[5] = 0.0f;  // reserved
+                x[5] = (float)log10(h_att * sqrt(rate2) + 1e-30);
                 x[6] = (float)log10((double)nm);
                 x[7] = (float)log10((double)d.nt_sys + 1.0);
                 x[8] = (float)log10(1.0 + (double)rej_before);
                 x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
                 x[10] = (float)ctl->prev_pc / 12.0f;
-                x[11] = 1.0f;
+                x[11] = (float)log10(A.mratio[sys] + 1e-12);
                 const float yl = mlp_eval(A.mlp, x, (float *)(E.red + 64));
                 if (tid == 0) flag[0] = (double)yl;
             }
@@ -584,6 +621,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 r[0] = d.sys_id; r[1] = t_att; r[2] = h_att; r[3] = estim; r[4] = err; r[5] = dtreq;
                 r[6] = accepted; r[7] = pc_used; r[8] = rej_before; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
                 r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = d.nt_sys; r[15] = h_next;
+                r[16] = rate2; r[17] = A.mratio[sys];
             }
         }
         if (accepted) {

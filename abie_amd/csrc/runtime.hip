@@ -113,6 +113,7 @@ struct abie_ensemble {
     DBuf<unsigned char> d_del;
     DBuf<unsigned long long> d_log_len;
     DBuf<float> d_mlp;
+    DBuf<double> d_mratio;             // per unit
     long long log_cap = 0;
     bool has_ext = false, has_mlp = false;
 };
@@ -246,6 +247,22 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     HIP_TRY(hipMemcpy(e->d_inv.p, inv.data(), T * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_del.p, del.data(), T, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_sys.p, e->desc.data(), U * sizeof(SysDesc), hipMemcpyHostToDevice));
+    // mass ratio of each unit's system: (sum of the other massive masses) / largest
+    {
+        std::vector<double> mr(U, 0.0);
+        for (int u = 0; u < U; ++u) {
+            const int s = e->unit_sys[u], o = offsets[s], n = offsets[s + 1] - o;
+            double mx = 0.0, sum = 0.0;
+            for (int i = 0; i < n; ++i)
+                if (m[o + i] > 0.0) {
+                    mx = std::max(mx, m[o + i]);
+                    sum += m[o + i];
+                }
+            mr[u] = mx > 0.0 ? (sum - mx) / mx : 0.0;
+        }
+        HIP_TRY(e->d_mratio.reserve(U));
+        HIP_TRY(hipMemcpy(e->d_mratio.p, mr.data(), U * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(e->d_order.p, e->order.data(), U * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(e->d_state.p, 0, (size_t)kStateDoubles * T * sizeof(double)));
     std::vector<SysCtl> ctl(U);
@@ -303,6 +320,7 @@ static GR15Args ens_args(abie_ensemble *e) {
     a.log_len = e->d_log_len.p;
     a.log_cap = e->log_cap;
     a.mlp = e->has_mlp ? e->d_mlp.p : nullptr;
+    a.mratio = e->d_mratio.p;
     a.max_attempts = e->max_attempts;
     a.events = e->events;
     return a;
@@ -393,7 +411,7 @@ void abie_ensemble_destroy(abie_ensemble *e) {
                     &e->d_ext, &e->d_state, &e->d_scan, &e->d_ring_ce, &e->d_ring_coll, &e->d_log, &e->d_energy})
         b->release();
     e->d_sys.release(); e->d_order.release(); e->d_orig.release(); e->d_inv.release();
-    e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release();
+    e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release(); e->d_mratio.release();
     for (auto &s : e->streams)
         if (s) (void)hipStreamDestroy(s);
     delete e;

@@ -13,11 +13,13 @@ accepts or rejects it, and a rejection falls back to the plain rule.
 
 Features (ABIE_MLP_IN = 12, computed identically on the device, see
 gr15_impl.hpp): log10(h_i/dtreq_{i-1}), log10 err_i, log10 err_{i-1},
-log10(h_i/h_{i-1}), pc_i/12, 0 (reserved), log10 n_massive,
-log10(n_test+1), log10(1 + rejections before step i),
-log10(dtreq_i/dtreq_{i-1}), pc_{i-1}/12, 1.  Every feature is invariant
-under a change of time scale: an absolute step size is out of distribution
-near close encounters, where a network that saw it could keep shrinking h.
+log10(h_i/h_{i-1}), pc_i/12, log10(h_i/tau) with tau = min over massive
+pairs of r_ij/|v_ij| (the shortest encounter timescale of the new state),
+log10 n_massive, log10(n_test+1), log10(1 + rejections before step i),
+log10(dtreq_i/dtreq_{i-1}), pc_{i-1}/12, log10 of the mass ratio (sum of
+the other massive masses / largest).  Every feature is invariant under a
+change of time unit: an absolute step size is out of distribution near
+close encounters, where a network that saw it could keep shrinking h.
 The device clamps the output to [-0.3, 0] dex: a proposal is never longer
 than the plain controller's dtreq.
 
@@ -31,7 +33,8 @@ from . import _lib
 
 EPS = 1e-300
 # step-log columns (gr15_impl.hpp)
-C_SYS, C_T, C_H, C_EST, C_ERR, C_DTREQ, C_ACC, C_PC, C_REJ, C_PH, C_PERR, C_PDT, C_PPC, C_NM, C_NT, C_HNEXT = range(16)
+(C_SYS, C_T, C_H, C_EST, C_ERR, C_DTREQ, C_ACC, C_PC, C_REJ, C_PH, C_PERR, C_PDT, C_PPC, C_NM, C_NT, C_HNEXT,
+ C_RATE2, C_MRATIO) = range(18)
 
 
 def features(rec):
@@ -45,13 +48,13 @@ def features(rec):
     x[:, 2] = np.log10(perr + EPS)
     x[:, 3] = np.log10(h / np.where(ph > 0, ph, h))
     x[:, 4] = r[:, C_PC] / 12.0
-    x[:, 5] = 0.0
+    x[:, 5] = np.log10(h * np.sqrt(r[:, C_RATE2]) + 1e-30)
     x[:, 6] = np.log10(r[:, C_NM])
     x[:, 7] = np.log10(r[:, C_NT] + 1.0)
     x[:, 8] = np.log10(1.0 + r[:, C_REJ])
     x[:, 9] = np.log10(np.where(pdt > 0, dtreq / np.where(pdt > 0, pdt, 1.0), 1.0))
     x[:, 10] = r[:, C_PPC] / 12.0
-    x[:, 11] = 1.0
+    x[:, 11] = np.log10(r[:, C_MRATIO] + 1e-12)
     return x.astype(np.float32)
 
 

@@ -198,9 +198,14 @@ int abie_ensemble_get_counters(abie_ensemble *e, long long *counters);
 /* Energies (common.c:340-357 per system) of the current state, [n_sys]. */
 int abie_ensemble_energy(abie_ensemble *e, double *energy);
 
-/* Attach / detach a step log: every attempt of every system appends one
- * record of ABIE_STEPLOG_FIELDS doubles while capacity lasts. */
-#define ABIE_STEPLOG_FIELDS 16
+/* Attach / detach a step log: every attempt of every system (its primary
+ * slice) appends one record of ABIE_STEPLOG_FIELDS doubles while capacity
+ * lasts: system, t, h, estimate, err, dtreq, accepted, PC iterations,
+ * rejections before, previous h / err / dtreq / PC iterations, massive
+ * bodies, test particles, next h, and for accepted attempts the largest
+ * |v_ij|^2 / r_ij^2 over massive pairs of the new state, and the system's
+ * mass ratio (sum of the other massive bodies) / (largest mass). */
+#define ABIE_STEPLOG_FIELDS 18
 int abie_ensemble_set_steplog(abie_ensemble *e, long long capacity);
 int abie_ensemble_get_steplog(abie_ensemble *e, double *records, long long *n_records);
 
