@@ -105,9 +105,9 @@ def train(X, y, epochs=300, lr=3e-3, batch=8192, seed=0, log_every=0):
     from . import engine
     mu = X.mean(0).astype(np.float64)
     sd = X.std(0).astype(np.float64)
-    sd[sd < 1e-6] = 1.0
-    mu[-1] = 0.0  # keep the constant input at 1
-    sd[-1] = 1.0
+    const = sd < 1e-6  # constant inputs pass through unchanged
+    mu[const] = 0.0
+    sd[const] = 1.0
     Z = ((X - mu) / sd).astype(np.float32)
     dev = "cuda"
     Zd = torch.tensor(Z, device=dev)

@@ -29,9 +29,14 @@ TRAIN_SEED = 20261203 + 500000
 EVAL_SEED = 20261203 + 900000
 
 
-def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None):
+def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None, weights=None):
+    """Step logs of n_sys seed systems under the massive-only controller, or
+    under the learned one driven by ``weights``."""
     systems = ics.ensemble(n_sys, seed=seed)
-    ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller="massive")
+    ens = engine.Ensemble(systems, G=G, force_mode=force_mode,
+                          controller="massive" if weights is None else "learned")
+    if weights is not None:
+        ens.set_proposer(weights)
     cap = cap or n_sys * 4000
     ens.set_steplog(cap)
     ens.run(years)
@@ -41,6 +46,14 @@ def collect_logs(n_sys, years, seed, force_mode="fp64", cap=None):
 
 
 CLAMP_LO, CLAMP_HI = -0.3, 0.0  # the device's clamp of the proposer output (gr15_impl.hpp)
+
+
+def split(rec):
+    """(train X, y, held-out X, y): systems with index % 10 == 9 held out."""
+    hold = rec[:, proposer.C_SYS] % 10 == 9
+    Xtr, ytr = proposer.dataset(rec[~hold])
+    Xte, yte = proposer.dataset(rec[hold])
+    return Xtr, ytr, Xte, yte
 
 
 def choose_margin(pred, y, grid=np.linspace(-0.2, 0.3, 51)):
@@ -96,32 +109,41 @@ def main():
     ap.add_argument("--eval-systems", type=int, default=256)
     ap.add_argument("--eval-years", type=float, default=10.0)
     ap.add_argument("--eval-seconds", type=float, default=120.0)
+    ap.add_argument("--dagger", type=int, default=2, help="rounds of data aggregation under the learned controller")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "proposer.npz"))
     args = ap.parse_args()
     t0 = time.perf_counter()
     rec = collect_logs(args.systems, args.years, TRAIN_SEED)
     print("step log: %d attempts from %d systems (%.1f s)" % (len(rec), args.systems, time.perf_counter() - t0),
           flush=True)
-    sys_ids = rec[:, proposer.C_SYS]
-    hold = sys_ids % 10 == 9
-    Xtr, ytr = proposer.dataset(rec[~hold])
-    Xte, yte = proposer.dataset(rec[hold])
+    Xtr, ytr, Xte, yte = split(rec)
     q = np.quantile(ytr, [0.001, 0.01, 0.5, 0.99, 0.999])
     print("samples: train %d, held out %d; target quantiles (0.1%%, 1%%, 50%%, 99%%, 99.9%%) %s"
           % (len(Xtr), len(Xte), np.array2string(q, precision=3)), flush=True)
-    w, (mu, sd), hist = proposer.train(Xtr, ytr, epochs=args.epochs, log_every=max(1, args.epochs // 6))
-    for ep, loss in hist:
-        print("epoch %d loss %.4e" % (ep, loss), flush=True)
-    pred = proposer.predict(w, Xte)
-    mse = float(np.mean((pred - yte) ** 2))
-    base = float(np.mean(yte ** 2))
-    margin, gain = choose_margin(pred, yte)
-    base_gain = float(np.mean(np.where(0.0 <= yte, 1.0, 0.0)))
-    print("held-out mse %.4e (predict-0 baseline %.4e); margin %.3f dex, step/attempt %.3f vs plain %.3f"
-          % (mse, base, margin, gain, base_gain), flush=True)
-    wm = proposer.with_margin(w, margin)
+    # Rounds of dataset aggregation: the learned controller visits states the
+    # massive-only controller's logs do not contain (h below dtreq, fewer
+    # rejections), so fresh systems are integrated with the current network
+    # and their steps added to the training set.
+    for rnd in range(args.dagger + 1):
+        w, (mu, sd), hist = proposer.train(Xtr, ytr, epochs=args.epochs, log_every=max(1, args.epochs // 6))
+        pred = proposer.predict(w, Xte)
+        mse = float(np.mean((pred - yte) ** 2))
+        base = float(np.mean(yte ** 2))
+        margin, gain = choose_margin(pred, yte)
+        base_gain = float(np.mean(np.where(0.0 <= yte, 1.0, 0.0)))
+        print("round %d: loss %.4e, held-out mse %.4e (predict-0 %.4e); margin %.3f dex, step/attempt %.3f vs plain %.3f"
+              % (rnd, hist[-1][1], mse, base, margin, gain, base_gain), flush=True)
+        wm = proposer.with_margin(w, margin)
+        if rnd == args.dagger:
+            break
+        rec2 = collect_logs(args.systems // 2, args.years, TRAIN_SEED + 100_000 * (rnd + 1), weights=wm)
+        X2, y2, X2e, y2e = split(rec2)
+        Xtr, ytr = np.concatenate([Xtr, X2]), np.concatenate([ytr, y2])
+        Xte, yte = np.concatenate([Xte, X2e]), np.concatenate([yte, y2e])
+        print("round %d: +%d samples under the learned controller" % (rnd + 1, len(X2)), flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     report = {"train_systems": args.systems, "train_years": args.years, "samples": int(len(Xtr)),
+              "dagger_rounds": args.dagger,
               "heldout_mse": mse, "heldout_mse_zero": base, "margin_dex": margin,
               "heldout_step_per_attempt": gain, "heldout_step_per_attempt_plain": base_gain,
               "target_quantiles": q.tolist(), "loss_history": hist}
