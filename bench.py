@@ -99,7 +99,44 @@ def cpu_baseline(n_test=65536, reps=3):
     dt = (time.perf_counter() - t0) / reps
     n_int = interactions(64, n_test)
     return {"value": n_int / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
-            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each" % (n_test, reps, dt)}
+            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each" % (n_test, reps, dt),
+            "host": host_info()}
+
+
+def host_info():
+    """nproc and CPU model of the host the CPU baselines run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_plummer(n=65536):
+    """The reference's C force loop (oracle/_ref) on the 65,536-body Plummer
+    sphere, one evaluation on one host core."""
+    import ctypes
+    from abie_amd import ics
+    ref = os.path.join(ROOT, "oracle", "_ref", "libabie_ref.so")
+    if not os.path.exists(ref):
+        return None
+    L = ctypes.CDLL(ref)
+    dp = ctypes.POINTER(ctypes.c_double)
+    f = L.ode_n_body_second_order
+    f.argtypes = [dp, ctypes.c_size_t, ctypes.c_double, dp, dp, dp]
+    pos, vel, m = ics.plummer(n)
+    p = np.ascontiguousarray(pos.reshape(-1))
+    acc = np.zeros_like(p)
+    r = np.zeros(n)
+    t0 = time.perf_counter()
+    f(p.ctypes.data_as(dp), n, 1.0, m.ctypes.data_as(dp), r.ctypes.data_as(dp), acc.ctypes.data_as(dp))
+    dt = time.perf_counter() - t0
+    return {"interactions_per_s": interactions(n, 0) / dt, "wall_s": dt, "cores": 1, "kind": "reference"}
 
 
 def gr15_dropin(years, force_mode="fp64"):
@@ -188,9 +225,11 @@ def host_aos_case(pos, m, reps=5):
     return dt
 
 
-def cpu_gr15_config0(years):
+def cpu_gr15_config0(years, chunk=None):
     """The reference's own GR15 (oracle/_ref, device -1) on the 1 star + 4
-    planets + 1,000 test particle system, one host core."""
+    planets + 1,000 test particle system, one host core; with ``chunk`` the
+    run is split into integrator_gr calls of that many years (as the
+    reference's Python driver does with store_dt), each one logged."""
     import ctypes
     from abie_amd import ics
     ref = os.path.join(ROOT, "oracle", "_ref", "libabie_ref.so")
@@ -209,12 +248,20 @@ def cpu_gr15_config0(years):
     r = np.zeros_like(mm)
     L.initialize_code(G, 0.0, len(m), 1, 1, -1)
     L.set_state(p.ctypes.data_as(dp), v.ctypes.data_as(dp), mm.ctypes.data_as(dp), r.ctypes.data_as(dp), len(m), G, 0.0)
+    step = chunk or years
+    t = 0.0
     t0 = time.perf_counter()
-    L.integrator_gr(0.0, years, 1.0)
+    while t < years:
+        L.integrator_gr(t, min(t + step, years), 1.0)
+        t = L.get_model_time()
+        if chunk:
+            log("cpu gr15: t = %.3f yr after %.1f s" % (t, time.perf_counter() - t0))
     wall = time.perf_counter() - t0
-    t = L.get_model_time()
     L.finalize_code()
-    return {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "cores": 1, "kind": "reference"}
+    out = {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "cores": 1, "kind": "reference"}
+    if chunk:
+        out["chunks_yr"] = chunk
+    return out
 
 
 def mixed_error_report(pos, m, Gc):
@@ -377,6 +424,11 @@ def main():
         log("config 2 run: %s" % extras["mixed_run_config2"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
         extras["ensemble_config3_4096x1yr"] = ensemble_report(4096, 1.0, w)
+        if not args.no_cpu:
+            extras["cpu_reference_plummer65536"] = cpu_plummer()
+            log("cpu plummer: %s" % extras["cpu_reference_plummer65536"])
+            extras["cpu_reference_gr15_config0_100yr"] = cpu_gr15_config0(100.0, chunk=10.0)
+            log("cpu gr15 100 yr: %s" % extras["cpu_reference_gr15_config0_100yr"])
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 8
         a = torch.ones(nel, dtype=torch.float64, device="cuda")
