@@ -257,7 +257,8 @@ def cpu_gr15_config0(years, chunk=None):
         if chunk:
             log("cpu gr15: t = %.3f yr after %.1f s" % (t, time.perf_counter() - t0))
     wall = time.perf_counter() - t0
-    L.finalize_code()
+    # no finalize_code: the reference initialises once per process
+    # (common.c:225-227), so a later call here must find its buffers alive
     out = {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "cores": 1, "kind": "reference"}
     if chunk:
         out["chunks_yr"] = chunk
