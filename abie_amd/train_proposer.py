@@ -1,6 +1,8 @@
 """Train the learned GR15 step proposer and measure what it gains.
 
-    python -m abie_amd.train_proposer [--systems 1024] [--years 20] [--out abie_amd/data/proposer.npz]
+    python -m abie_amd.train_proposer [--systems 1024] [--years 3] [--dagger 2] [--out abie_amd/data/proposer.npz]
+
+(the defaults are the settings of the committed abie_amd/data/proposer.npz)
 
 1. Step logs: ``--systems`` seed systems (drawn like the ensemble workload:
    massive U{3..64}, test particles U{0..2048}, seeds disjoint from the
@@ -12,7 +14,9 @@
    the systems; the other 10 % are held out.
 4. Margin: the output shift that maximises predicted step length per
    attempt on the held-out samples (a rejected attempt costs its step).
-5. Evaluation: a fresh set of systems integrated with and without the
+5. Aggregation (``--dagger`` rounds): fresh systems are integrated with the
+   current network, their steps join the data set, and 3-4 are repeated.
+6. Evaluation: a fresh set of systems integrated with and without the
    proposer; evaluations per simulated year, rejection rate, |dE/E|.
 """
 import argparse
@@ -104,11 +108,11 @@ def evaluate(weights, n_sys, years, force_mode="fp64", max_seconds=120.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--systems", type=int, default=1024)
-    ap.add_argument("--years", type=float, default=20.0)
+    ap.add_argument("--years", type=float, default=3.0)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--eval-systems", type=int, default=256)
-    ap.add_argument("--eval-years", type=float, default=10.0)
-    ap.add_argument("--eval-seconds", type=float, default=120.0)
+    ap.add_argument("--eval-years", type=float, default=2.0)
+    ap.add_argument("--eval-seconds", type=float, default=60.0)
     ap.add_argument("--dagger", type=int, default=2, help="rounds of data aggregation under the learned controller")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "proposer.npz"))
     args = ap.parse_args()
