@@ -103,6 +103,23 @@ def cpu_baseline(n_test=65536, reps=3):
             "host": host_info()}
 
 
+def cpu_all_cores(n_test=262144, reps=2):
+    """The oracle's restatement of the same force loop with the sinks split
+    over every host thread of the box (at most 16): a reported multi-core
+    CPU rate beside the single-core reference baseline."""
+    from abie_amd import ics
+    from oracle import oracle
+    k = max(1, min(16, os.cpu_count() or 1))
+    pos, vel, m = ics.config1a(n_test=n_test)
+    oracle.accel_threads(pos, G, m, k)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.accel_threads(pos, G, m, k)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": interactions(64, n_test) / dt, "unit": "interactions/s", "cores": k, "kind": "port",
+            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each" % (n_test, reps, dt)}
+
+
 def host_info():
     """nproc and CPU model of the host the CPU baselines run on."""
     model = None
@@ -442,6 +459,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         log("cpu: %s" % out["cpu_baseline"])
+        out["extras"]["cpu_restatement_all_cores"] = cpu_all_cores()
+        log("cpu, all cores: %s" % out["extras"]["cpu_restatement_all_cores"])
     else:
         out["cpu_baseline"] = None
     import ctypes

@@ -39,7 +39,15 @@ double orc_div_const(double x, double c, double rc)
  * are left untouched. */
 void orc_accel(const double *pos, size_t n, double G, const double *m, double *acc)
 {
-    for (size_t j = 0; j < n; j++) {
+    orc_accel_range(pos, n, G, m, acc, 0, n);
+}
+
+/* The same loop for the sinks [j0, j1) only (every source still counts), so
+ * that sink ranges can run on separate host threads; per sink the arithmetic
+ * is identical to orc_accel. */
+void orc_accel_range(const double *pos, size_t n, double G, const double *m, double *acc, size_t j0, size_t j1)
+{
+    for (size_t j = j0; j < j1 && j < n; j++) {
         if (m[j] < 0.0) continue;
         const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
         double ax = 0.0, ay = 0.0, az = 0.0;

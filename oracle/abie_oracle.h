@@ -61,6 +61,8 @@ typedef struct orc_sim {
 
 /* common.c:39-77  ode_n_body_second_order */
 void orc_accel(const double *pos, size_t n, double G, const double *m, double *acc);
+/* orc_accel for the sinks [j0, j1) (host-thread sharding of the CPU baseline). */
+void orc_accel_range(const double *pos, size_t n, double G, const double *m, double *acc, size_t j0, size_t j1);
 
 /* common.c:82-114 calculate_accelerations (CPU branch; C>0 is rejected) */
 int orc_calculate_accelerations(orc_sim *s, const double *pos, const double *vel, double *acc);

@@ -80,6 +80,9 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_accel.argtypes = [_dp, ctypes.c_size_t, ctypes.c_double, _dp, _dp]
         L.orc_accel.restype = None
+        L.orc_accel_range.argtypes = [_dp, ctypes.c_size_t, ctypes.c_double, _dp, _dp, ctypes.c_size_t,
+                                      ctypes.c_size_t]
+        L.orc_accel_range.restype = None
         L.orc_gr15.argtypes = [ctypes.POINTER(OrcSim), ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_gr15.restype = ctypes.c_int
         L.orc_energy.argtypes = [ctypes.POINTER(OrcSim)]
@@ -111,6 +114,23 @@ def accel(pos, G, m, acc_init=None):
     n = m.size
     acc = np.zeros(3 * n) if acc_init is None else np.array(acc_init, dtype=np.float64).reshape(-1)
     lib().orc_accel(_ptr(pos), n, G, _ptr(m), _ptr(acc))
+    return acc.reshape(n, 3)
+
+
+def accel_threads(pos, G, m, threads):
+    """``accel`` with the sinks split into ``threads`` contiguous ranges, each
+    on its own host thread (ctypes releases the GIL during the C call); the
+    result is bit-identical to ``accel``."""
+    from concurrent.futures import ThreadPoolExecutor
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    n = m.size
+    acc = np.zeros(3 * n)
+    L = lib()
+    bounds = np.linspace(0, n, threads + 1).astype(np.int64)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda k: L.orc_accel_range(_ptr(pos), n, G, _ptr(m), _ptr(acc), int(bounds[k]),
+                                                int(bounds[k + 1])), range(threads)))
     return acc.reshape(n, 3)
 
 

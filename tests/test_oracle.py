@@ -267,3 +267,14 @@ def test_glibc_pow_is_not_always_correctly_rounded():
     from oracle import oracle as O
     assert O._cr_pow(x, 1.0 / 7) == cr
     assert math.pow(x, 1.0 / 7) in (cr, math.nextafter(cr, 0), math.nextafter(cr, 2))
+
+
+def test_threaded_sink_ranges_bit_identical(orc):
+    """The multi-core CPU baseline splits sinks over host threads; every
+    acceleration is bit-identical to the single-threaded loop."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(3, 9, 500)
+    m[7] = -1.0  # a deleted sink stays zero
+    a = orc.accel(pos, G, m)
+    b = orc.accel_threads(pos, G, m, 5)
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
