@@ -255,6 +255,7 @@ struct Engine {
     int buf = 0;
     Coef2 kc;
     int slot_valid[BPT], slot_massive[BPT], slot_del[BPT], slot_orig[BPT];
+    double gm0;  // G m of this thread's slot-0 body when massive (massive bodies sit in slot 0)
     bool wave_massive;
 
     __device__ Engine(const GR15Args &a, int s, const SysDesc &d, double *lds)
@@ -270,6 +271,7 @@ struct Engine {
             slot_del[k] = (b < n) ? (int)A.deleted[body0 + b] : 0;
             slot_orig[k] = (b < n) ? A.orig[body0 + b] : 0;
         }
+        gm0 = ((int)threadIdx.x < nm) ? A.gm[body0 + threadIdx.x] : 0.0;
     }
 
     // accelerations at positions p for every owned body; one barrier
@@ -283,7 +285,7 @@ struct Engine {
                 S[4 * b] = p[k][0];
                 S[4 * b + 1] = p[k][1];
                 S[4 * b + 2] = p[k][2];
-                S[4 * b + 3] = A.gm[body0 + b];
+                S[4 * b + 3] = gm0;  // b < nm <= NT: slot 0
             }
         }
         __syncthreads();
