@@ -12,10 +12,12 @@ namespace abie {
 // rest, ascending).  nm = number of massive bodies (m > 0).  A system split
 // into several units (test-particle slices) has one SysDesc per unit: nt_sys
 // is the whole system's test-particle count, sys_id its index and primary is
-// 1 on the unit whose massive bodies are reported.
+// 1 on the unit whose massive bodies are reported.  norm_all = 1 makes a
+// massive-only controller use every body in its norms (systems with at most
+// one massive body, whose massive norm would be 0/0).
 struct SysDesc {
     int body0, n, nm, cfg;
-    int nt_sys, sys_id, primary, pad;
+    int nt_sys, sys_id, primary, norm_all;
 };
 
 // Per-system control block, device-resident; persists across launches.

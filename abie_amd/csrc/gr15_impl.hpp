@@ -256,6 +256,7 @@ struct Engine {
     Coef2 kc;
     int slot_valid[BPT], slot_massive[BPT], slot_del[BPT], slot_orig[BPT];
     double gm0;  // G m of this thread's slot-0 body when massive (massive bodies sit in slot 0)
+    bool norm_all;
     bool wave_massive;
 
     __device__ Engine(const GR15Args &a, int s, const SysDesc &d, double *lds)
@@ -272,6 +273,7 @@ struct Engine {
             slot_orig[k] = (b < n) ? A.orig[body0 + b] : 0;
         }
         gm0 = ((int)threadIdx.x < nm) ? A.gm[body0 + threadIdx.x] : 0.0;
+        norm_all = d.norm_all != 0;
     }
 
     // accelerations at positions p for every owned body; one barrier
@@ -321,7 +323,7 @@ struct Engine {
 
     // which components enter the controller's norms
     __device__ __forceinline__ bool in_norm(int k) const {
-        return CT == ABIE_CTRL_PLAIN ? (bool)slot_valid[k] : (bool)slot_massive[k];
+        return (CT == ABIE_CTRL_PLAIN || norm_all) ? (bool)slot_valid[k] : (bool)slot_massive[k];
     }
 };
 
@@ -412,7 +414,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         for (int k = 0; k < BPT; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (CT == ABIE_CTRL_PLAIN ? (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n) : (bool)E.slot_massive[k]) {
+                if (CT == ABIE_CTRL_PLAIN ? (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n)
+                                          : (E.norm_all ? (bool)E.slot_valid[k] : (bool)E.slot_massive[k])) {
                     d0 = fmax(d0, fabs(y0[k][c]));
                     d1 = fmax(d1, fabs(a0[k][c]));
                 }
@@ -509,7 +512,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                     ma = fmax(ma, fabs(acc[k][c]));
                 }
         wg_max2<NT>(mb, ma, E.red);
-        const double estim = mb / ma;
+        // a system without any acceleration (one body alone, no ext-acc)
+        // would give 0/0 as in the reference; it moves freely: estim 0
+        const double estim = (ma > 0.0) ? mb / ma : 0.0;
         const double err = pow_pos(estim / kEpsb, 1. / 7);
         const double dtreq = h / err;
         const double h_att = h, t_att = t;
@@ -579,6 +584,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         else h_next_plain = dtreq;
         double h_next = h_next_plain;
         double q = dtreq / h_att;  // refine_bs ratio of the reference
+        if (q > DBL_MAX) q = 1.0 / kFac;  // only when estim = 0 (free motion, all b = 0)
 
         // the learned proposer (after accepted steps only)
         if (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr) {

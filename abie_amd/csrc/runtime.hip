@@ -145,6 +145,9 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         const int nm = engine_order(m + o, n, perms[s]);
         nms[s] = nm;
         const int nt = n - nm;
+        // (with at most one massive body the slices control their steps on
+        // their own bodies, norm_all; the lone star then moves at constant
+        // velocity in each slice, equal to rounding)
         const bool slice = e->ctrl != ABIE_CTRL_PLAIN && n > cap && nm <= 512 && cap - nm >= 64;
         if (!slice && pick_cfg(n, nm) >= 0) {
             units.push_back({s, nm, nm, n});
@@ -181,7 +184,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         if (first) e->sys_unit0[s] = u;
         e->unit_sys[u] = s;
         const int n = un.nm + (un.t1 - un.t0);
-        e->desc[u] = SysDesc{(int)g0, n, un.nm, pick_cfg(n, un.nm), (offsets[s + 1] - o) - un.nm, s, first ? 1 : 0, 0};
+        e->desc[u] = SysDesc{(int)g0, n, un.nm, pick_cfg(n, un.nm), (offsets[s + 1] - o) - un.nm, s, first ? 1 : 0,
+                              un.nm <= 1 ? 1 : 0};
         for (int p = 0; p < n; ++p) {
             const int ep = p < un.nm ? p : un.t0 + (p - un.nm);  // engine position in the whole system
             const int i = perms[s][ep];

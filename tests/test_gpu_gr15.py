@@ -164,3 +164,57 @@ def test_ensemble_energy_conservation_mixed():
     ens.run(2.0)
     e1 = ens.energy()
     assert np.all(np.abs((e1 - e0) / e0) < 1e-12)
+
+
+def kepler_position(r0, v0, mu, t):
+    """Universal-variable two-body solution (mpmath, 40 digits)."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    r0 = [mp.mpf(float(x)) for x in r0]; v0 = [mp.mpf(float(x)) for x in v0]
+    mu = mp.mpf(mu)
+    rn = mp.sqrt(sum(x * x for x in r0)); vn2 = sum(x * x for x in v0)
+    a = 1 / (2 / rn - vn2 / mu)
+    n = mp.sqrt(mu / a ** 3)
+    rv = sum(x * y for x, y in zip(r0, v0))
+    e_cos = 1 - rn / a; e_sin = rv / mp.sqrt(mu * a)
+    e = mp.sqrt(e_cos ** 2 + e_sin ** 2)
+    E0 = mp.atan2(e_sin, e_cos)
+    M = E0 - e_sin + n * mp.mpf(t)
+    E = M
+    for _ in range(60):
+        E = E - (E - e * mp.sin(E) - M) / (1 - e * mp.cos(E))
+    dE = E - E0
+    F = 1 - a / rn * (1 - mp.cos(dE))
+    Gt = mp.mpf(t) - mp.sqrt(a ** 3 / mu) * (dE - mp.sin(dE))
+    return np.array([float(F * r0[i] + Gt * v0[i]) for i in range(3)])
+
+
+@pytest.mark.parametrize("controller", ["plain", "massive"])
+def test_kepler_orbit_closed_form(controller):
+    """A test particle around a star (one massive body: the massive-only
+    controller then controls on every body) for 10 orbits of e ~ 0.21,
+    against the closed-form solution."""
+    from abie_amd import engine
+    r0 = np.array([1.0, 0.0, 0.0]); v0 = np.array([0.0, 2 * np.pi * 1.1, 0.0])
+    pos = np.array([[0.0, 0.0, 0.0], r0]); vel = np.array([[0.0, 0.0, 0.0], v0])
+    m = np.array([1.0, 0.0])
+    ens = engine.Ensemble([(pos, vel, m)], G=G, controller=controller)
+    assert ens.run(10.0)
+    p, v, t, st = ens.state()
+    want = kepler_position(r0, v0, G, t[0])
+    assert np.all(np.isfinite(p)) and st[0] == 0
+    assert np.linalg.norm(p[1] - want) < 1e-9
+    assert np.array_equal(p[0], np.zeros(3))
+
+
+def test_force_free_body_moves_freely():
+    """One body alone has no acceleration (the reference's error estimate
+    would be 0/0): it moves at constant velocity and the run ends."""
+    from abie_amd import engine
+    pos = np.array([[1.0, 2.0, 3.0]]); vel = np.array([[0.5, -0.25, 0.125]])
+    for ctrl in ("plain", "massive"):
+        ens = engine.Ensemble([(pos, vel, np.array([1.0]))], G=G, controller=ctrl)
+        assert ens.run(2.0)
+        p, v, t, st = ens.state()
+        assert t[0] >= 2.0 and np.allclose(p[0], pos[0] + vel[0] * t[0], rtol=1e-14, atol=1e-14)
+        assert np.array_equal(v[0], vel[0])
