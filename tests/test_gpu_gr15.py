@@ -218,3 +218,20 @@ def test_force_free_body_moves_freely():
         p, v, t, st = ens.state()
         assert t[0] >= 2.0 and np.allclose(p[0], pos[0] + vel[0] * t[0], rtol=1e-14, atol=1e-14)
         assert np.array_equal(v[0], vel[0])
+
+
+def test_exact_gr15_with_deleted_and_massless_bodies(orc):
+    """A deleted body (m < 0: neither source nor sink) and test particles in
+    one system: the device GR15 with exact forces equals the oracle bit for
+    bit (both give the deleted body zero acceleration)."""
+    from abie_amd import engine, ics
+    pos, vel, m = ics.planetary_system(31, 6, 20)
+    m = m.copy()
+    m[3] = -1.0
+    ens = engine.Ensemble([(pos, vel, m)], G=G, force_mode="exact", controller="plain")
+    assert ens.run(0.4)
+    p, v, t, _ = ens.state()
+    s = orc.Sim(pos, vel, m, cr_pow=True)
+    s.integrate(0.4, 0.0)
+    assert t[0] == s.t
+    assert same_bits(p, s.pos.reshape(-1, 3)) and same_bits(v, s.vel.reshape(-1, 3))
