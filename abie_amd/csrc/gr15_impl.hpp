@@ -485,6 +485,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
     double dy[BPT][3];
     while (n_att < A.max_attempts) {
         int pc_used = 0;
+        double pc_last = DBL_MAX;
         for (int ipc = 0; ipc < 12; ++ipc) {
             pc_used = ipc + 1;
             double mdb = -DBL_MAX, mf = -DBL_MAX;
@@ -512,7 +513,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         mf = fmax(mf, fabs(acc[k][c]));
                     }
             wg_max2<NT>(mdb, mf, E.red);
-            if (mdb / mf < kTolpc) break;
+            const double pc_err = mdb / mf;
+            if (pc_err < kTolpc) break;
+            // massive-only controllers: also stop once the correction no
+            // longer shrinks (its round-off floor), as IAS15 does (Rein &
+            // Spiegel 2015, MNRAS 446, 1424); the plain controller iterates
+            // to 1e-16 or 12 times like the reference
+            if (kFast && ipc >= 2 && pc_err >= pc_last) break;
+            pc_last = pc_err;
 #pragma unroll
             for (int k = 0; k < BPT; ++k)
 #pragma unroll
