@@ -23,6 +23,7 @@ from .data_io import DataIO
 from .events import CloseEncounterException, CollisionException  # noqa: F401
 from .integrator import Integrator
 from .particles import Particle, Particles  # noqa: F401
+from .recorder import SimulationDataRecorder
 
 __all__ = ["ABIE", "Particles", "Particle", "Integrator", "CollisionException", "CloseEncounterException"]
 
@@ -169,19 +170,38 @@ class ABIE(object):
 
     @property
     def data(self):
+        """Flush the snapshot buffer and return what the recorder holds
+        (abie/__init__.py:266-269)."""
         self.buffer.flush()
+        return self.buffer.recorder.data
+
+    def record_simulation(self, particles=None, quantities=None):
+        """Choose what the recorder keeps and return it; recording begins at
+        its ``start()`` (abie/__init__.py:271-280)."""
+        if self.buffer.recorder is None:
+            self.buffer.recorder = SimulationDataRecorder(particles, quantities)
+        else:
+            self.buffer.recorder.set_monitored_particles(particles)
+            self.buffer.recorder.set_monitored_quantities(quantities)
         return self.buffer.recorder
 
     # ------------------------------------------------------------ actions
     def initialize(self, config=None):
         if self.__integrator is None:
             self.integrator = self.__integrator_name
-        if config is not None:
-            self.integrator.CONST_G = float(config["physical_params"]["G"])
-            self.integrator = config["integration"]["integrator"]
-            self.integrator.h = float(config["integration"]["h"])
-            self.integrator.t_start = float(config["integration"].get("t0", 0.0))
+        if config is None:
+            self.integrator.initialize()
+            return
+        # a parsed configuration file (abie/__init__.py:297-325)
+        self.CONST_G = float(config["physical_params"]["G"])
+        self.integrator = config["integration"]["integrator"]
+        self.integrator.h = float(config["integration"]["h"])
+        self.integrator.acceleration_method = config["integration"].get("acc_method", "ctypes")
+        if self.integrator.t_start == 0:
+            self.integrator.t_start = float(config["integration"]["t0"])
+        if self.t_end == 0:
             self.t_end = float(config["integration"]["tf"])
+        DataIO.ic_populate(config["initial_conds"], self, names=config.get("names"))
         self.integrator.initialize()
 
     def stop(self):
@@ -230,6 +250,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("-c", "--config", help="config file (TOML)", default=None)
     ap.add_argument("-o", "--output_file", dest="output_file", default="data.hdf5")
+    ap.add_argument("-r", "--rebound_file", help="REBOUND simulation file", default=None)
     ap.add_argument("-t", "--t_end", type=float, dest="t_end")
     ap.add_argument("-d", "--dt", type=float, dest="dt", default=None)
     ap.add_argument("-s", "--store_dt", type=float, dest="store_dt", default=100)
@@ -242,7 +263,9 @@ def main():
     if args.t_end is not None:
         sim.t_end = args.t_end
     if args.config is not None:
-        import tomllib  # noqa: F401  (Python >= 3.11)
+        sim.initialize(DataIO.parse_config_file(args.config))
+    elif args.rebound_file is not None:
+        DataIO.ic_populate_from_rebound(args.rebound_file, sim)
     if args.dt is not None:
         sim.h = args.dt
     sim.store_dt = args.store_dt

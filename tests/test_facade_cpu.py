@@ -51,14 +51,3 @@ def test_plugins_keep_reference_names():
     from abie_amd.integrator import Integrator
     names = set(Integrator.load_integrators())
     assert names >= {"GaussRadau15", "WisdomHolman", "RungeKutta", "AdamsBashforth", "LeapFrog", "Euler"}
-
-
-def test_dataio_flush(tmp_path):
-    from abie_amd.data_io import DataIO
-    d = DataIO(buf_len=3, output_file_name=str(tmp_path / "sim.hdf5"))
-    for k in range(5):
-        d.store_state(float(k), np.arange(6.0), np.ones(6), np.array([1.0, 0.0]), energy=-1.0)
-    d.close()
-    z = np.load(tmp_path / "sim.npz")
-    assert len([k for k in z.files if k.endswith("/t")]) == 5
-    assert z["Step#4/t"] == 4.0
