@@ -78,22 +78,24 @@ struct Interact<ABIE_FORCE_FP64_EXACT> {
     }
 };
 
-// Test particle from a massive body: FP64 relative coordinates, FP32 r^-3 by
-// v_rsq_f32 and one Newton step, FP64 accumulation of the FP32 term.
+// Test particle from a massive body at single-precision accuracy: the
+// hardware reciprocal square root (relative error below 2^-23, measured
+// 2^-24.2) used without its correction terms, so r^-3 carries a relative
+// error of about 3 * 2^-24.  FP32 arithmetic would not be cheaper: without
+// packed FP32 an FP32 op issues at the FP64 rate, and converting the
+// coordinates and results costs more ops than the correction saved here
+// (12 VALU ops + 1 v_rsq_f64 per pair against 16 + 1 for FP64 mode).
 template <>
 struct Interact<ABIE_FORCE_MIXED> {
     static __device__ __forceinline__ void run(double px, double py, double pz, double sx, double sy,
                                                double sz, double gm, double &ax, double &ay, double &az, Coef) {
-        const float dx = (float)(px - sx), dy = (float)(py - sy), dz = (float)(pz - sz);
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        float y = __builtin_amdgcn_rsqf(r2);
-        const float hy = 0.5f * y;
-        const float e = fmaf(-r2 * y, y, 1.0f);
-        y = fmaf(hy, e, y);
-        const float s = ((float)gm * y) * (y * y);
-        ax = ax - (double)(s * dx);
-        ay = ay - (double)(s * dy);
-        az = az - (double)(s * dz);
+        const double dx = px - sx, dy = py - sy, dz = pz - sz;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double y = __builtin_amdgcn_rsq(r2);
+        const double s = (gm * y) * (y * y);
+        ax = fma(-s, dx, ax);
+        ay = fma(-s, dy, ay);
+        az = fma(-s, dz, az);
     }
 };
 

@@ -203,16 +203,13 @@ __device__ __forceinline__ void pair(double px, double py, double pz, const doub
         ax = fma(-t, dx, ax);
         ay = fma(-t, dy, ay);
         az = fma(-t, dz, az);
-    } else {  // MIXED: FP32 r^-3 from FP64 relative coordinates
-        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-        const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-        float y = __builtin_amdgcn_rsqf(r2);
-        const float e = fmaf(-r2 * y, y, 1.0f);
-        y = fmaf(0.5f * y, e, y);
-        const float t = ((float)s[3] * y) * (y * y);
-        ax = ax - (double)(t * fx);
-        ay = ay - (double)(t * fy);
-        az = az - (double)(t * fz);
+    } else {  // MIXED: uncorrected hardware rsq, single-precision accuracy
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double y = __builtin_amdgcn_rsq(r2);
+        const double t = (s[3] * y) * (y * y);
+        ax = fma(-t, dx, ax);
+        ay = fma(-t, dy, ay);
+        az = fma(-t, dz, az);
     }
 }
 

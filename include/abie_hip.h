@@ -123,9 +123,10 @@ size_t ode_n_body_first_order(double *pos, size_t N, double G, const double *mas
 #define ABIE_FORCE_FP64       0  /* rsqrt seed + Taylor correction, FMA accumulate */
 #define ABIE_FORCE_FP64_EXACT 1  /* sqrt + divide, reference operation order:
                                     bit-identical to src/common.c:59-71          */
-#define ABIE_FORCE_MIXED      2  /* massive->massive FP64; massive->test FP32
-                                    rsqrt + Newton, FP64 relative coordinates
-                                    and FP64 accumulation                         */
+#define ABIE_FORCE_MIXED      2  /* massive->massive FP64; massive->test at
+                                    single-precision accuracy (hardware rsqrt
+                                    without correction, ~3*2^-24 relative);
+                                    FP64 coordinates and accumulation            */
 
 /* Last error message of this thread's most recent failing call. */
 const char *abie_hip_last_error(void);
