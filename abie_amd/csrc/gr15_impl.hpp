@@ -381,6 +381,30 @@ __device__ __forceinline__ float mlp_eval(const float *__restrict__ w, const flo
     return o + b3[0];
 }
 
+// Close-encounter / collision filter of one body against LDS tile entries
+// [q0, cnt): true if some pair passes r^2 <= max(cd^2, (r_j + r_q)^2) with a
+// 2^-46 relative slack, a superset of the exact test's d <= cd or d <= r_j +
+// r_q: d = sqrt(r^2) rounds, and r^2 with FMAs and the squared thresholds
+// differ from the exact pass's values by a few ulp (2^-50 would be tight).  RAD = false:
+// no body of the unit has a radius, the threshold is cd^2 for every pair.
+// self >= 0: the diagonal tile, only entries q > self count.
+template <bool RAD>
+__device__ __forceinline__ bool scan_tile(const double *tile, int q0, int cnt, int self, const double (&y)[3],
+                                          double rj, double thr0, double cd2) {
+    bool any = false;
+    for (int q = q0; q < cnt; ++q) {
+        const double dx = y[0] - tile[4 * q], dy = y[1] - tile[4 * q + 1], dz = y[2] - tile[4 * q + 2];
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        double thr = thr0;
+        if (RAD) {
+            const double rr = rj + tile[4 * q + 3];
+            thr = (rr > 0 && thr0 >= 0) ? fmax(cd2, rr * rr * (1.0 + 0x1p-46)) : thr0;
+        }
+        any |= (r2 <= thr) && q > self;
+    }
+    return any;
+}
+
 // WPE: waves per SIMD the register allocation targets (512 / WPE registers
 // per lane, AGPRs included): 1 lets a 256-thread group keep 4 bodies per
 // thread in VGPRs + AGPRs; 3 for the one-body-per-thread shapes of 128 to
@@ -423,6 +447,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             a0[k][c] = A.state[(long long)(42 + c) * T + gi];
             b6p[k][c] = A.state[(long long)(45 + c) * T + gi];
         }
+    }
+    // events: does any body of the unit have a radius (collisions possible)?
+    bool any_rad = false;
+    if (A.events) {
+        double rmax = 0.0, rdummy = 0.0;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const int b = k * NT + tid;
+            if (b < n) rmax = fmax(rmax, A.rad[b0 + b]);
+        }
+        wg_max2<NT>(rmax, rdummy, E.red);
+        any_rad = rmax > 0.0;
     }
     double t = ctl->t, h = ctl->h;
     const double t_end = ctl->t_end;
@@ -692,24 +728,45 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         sx[b0 + b] = y0[k][0]; sy[b0 + b] = y0[k][1]; sz[b0 + b] = y0[k][2];
                     }
                 }
-                __syncthreads();
+                // hit detection: a superset test over each unordered pair
+                // once, bodies staged in LDS tiles of NT (x, y, z, radius);
+                // a hit runs the exact pass below.  The test is symmetric in
+                // the pair, so checking i > j only finds the same hits.
                 const double cd = A.ce_dist;
-                const double cd2 = cd * cd * (1.0 + 0x1p-50);
+                const double cd2 = cd * cd * (1.0 + 0x1p-46);
+                double *tile = E.src;  // the force source list is not live here
+                const int lane0 = tid & ~63;
+                bool any = false;
                 double hit = 0.0, dummy2 = 0.0;
+                for (int kt = 0; kt < BPT && kt * NT < n; ++kt) {
+                    const int base = kt * NT;
+                    const int cnt = min(NT, n - base);
+                    __syncthreads();
+                    if (tid < cnt) {
+                        tile[4 * tid] = y0[kt][0];
+                        tile[4 * tid + 1] = y0[kt][1];
+                        tile[4 * tid + 2] = y0[kt][2];
+                        tile[4 * tid + 3] = A.rad[b0 + base + tid];
+                    }
+                    __syncthreads();
 #pragma unroll
-                for (int k = 0; k < BPT; ++k) {
-                    const int j = k * NT + tid;
-                    if (j >= n) continue;
-                    const double rj = A.rad[b0 + j];
-                    for (int kk = 0; kk < n; ++kk) {
-                        if (kk == j) continue;
-                        const double dx = y0[k][0] - sx[b0 + kk], dyy = y0[k][1] - sy[b0 + kk],
-                                     dz = y0[k][2] - sz[b0 + kk];
-                        const double r2 = dx * dx + dyy * dyy + dz * dz;
-                        const double rr = rj + A.rad[b0 + kk];
-                        if (r2 <= cd2 || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50))) hit = 1.0;
+                    for (int k = 0; k < BPT; ++k) {
+                        if (k > kt || k * NT + lane0 >= n) continue;  // tile wholly below this wave's bodies
+                        const int j = k * NT + tid;
+                        const bool valid = j < n;
+                        const double rj = valid ? A.rad[b0 + j] : 0.0;
+                        const double thr0 = valid ? cd2 : -1.0;  // -1: no pair of an idle lane passes
+                        // on the diagonal tile only i = base + q > j: the wave's
+                        // first lane bounds the loop, the rest is a lane mask
+                        const bool diag = k == kt;
+                        const int q0 = diag ? lane0 + 1 : 0;
+                        if (any_rad)
+                            any |= scan_tile<true>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                        else
+                            any |= scan_tile<false>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
                     }
                 }
+                hit = any ? 1.0 : 0.0;
                 wg_max2<NT>(hit, dummy2, E.red);
                 if (hit > 0.0) {
                     // exact pass in the reference's (j, k) order, one thread

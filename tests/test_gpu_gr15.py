@@ -235,3 +235,76 @@ def test_exact_gr15_with_deleted_and_massless_bodies(orc):
     s.integrate(0.4, 0.0)
     assert t[0] == s.t
     assert same_bits(p, s.pos.reshape(-1, 3)) and same_bits(v, s.vel.reshape(-1, 3))
+
+
+@pytest.mark.parametrize("pair", [(3, 900), (600, 601), (700, 1100), (1190, 1195)])
+def test_close_encounter_scan_across_tiles_matches_oracle(clib, orc, pair):
+    """integrator_gr on 2 massive bodies + 1,200 test particles, one pair of
+    test particles kept within the close-encounter distance (the scan stages
+    the 1,202 bodies in three LDS tiles of 512; the pairs sit across tiles
+    and on a tile's diagonal): return code, time, state and ring bit-identical
+    to the oracle's GR15 with exact forces."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(909, 2, 1200)
+    i, j = pair
+    pos = pos.copy()
+    vel = vel.copy()
+    pos[j] = pos[i] + np.array([3e-4, -2e-4, 1e-4])
+    vel[j] = vel[i]
+    n = len(m)
+    r = np.zeros(n)
+    clib.initialize_code(G, 0.0, n, 2, 1, 1e-3, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, 0.5, 1.0)
+    except ParticleException as ex:
+        ret = 1 if "Close" in type(ex).__name__ else 2
+    clib.get_state(p, v, mm, rr)
+    s = orc.Sim(pos, vel, m, r, max_ce=2, max_coll=1, ce_dist=1e-3, cr_pow=True)
+    ref = s.integrate(0.5, 0.0)
+    assert ret == ref == 1
+    assert clib.get_model_time() == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    ce = clib.get_close_encounter_data().reshape(-1)
+    assert same_bits(ce, s.buf_ce)
+    assert sorted(ce[1:3]) == [i, j]
+
+
+@pytest.mark.parametrize("pair", [(5, 1000), (1100, 1101)])
+def test_collision_scan_with_radii_across_tiles_matches_oracle(clib, orc, pair):
+    """The same 1,202-body system with radii on every body (the scan's
+    radius path): the close pair collides on the first accepted step; return
+    code, time, state and both rings as the oracle's GR15."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(910, 2, 1200)
+    i, j = pair
+    pos = pos.copy()
+    vel = vel.copy()
+    pos[j] = pos[i] + np.array([3e-4, -2e-4, 1e-4])
+    vel[j] = vel[i]
+    n = len(m)
+    r = np.random.default_rng(4).uniform(1e-7, 1e-6, n)
+    r[i] = r[j] = 2e-4
+    clib.initialize_code(G, 0.0, n, 2, 1, 1e-3, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, 0.5, 1.0)
+    except ParticleException as ex:
+        ret = 1 if "Close" in type(ex).__name__ else 2
+    clib.get_state(p, v, mm, rr)
+    s = orc.Sim(pos, vel, m, r, max_ce=2, max_coll=1, ce_dist=1e-3, cr_pow=True)
+    ref = s.integrate(0.5, 0.0)
+    assert ret == ref == 2
+    assert clib.get_model_time() == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
+    coll = clib.get_collision_data().reshape(-1)
+    assert same_bits(coll, s.buf_coll) and sorted(coll[1:3]) == [i, j]
