@@ -213,26 +213,29 @@ __device__ __forceinline__ void pair(double px, double py, double pz, const doub
     }
 }
 
-// The same pair with the self-pair (source index == own) removed.
+// The same pair with the self-pair (source index == own) removed: 1.0 is
+// added to r^2 of the self-pair only (d = 0 there, so its term is an exact
+// zero and acc is unchanged; for every other pair r^2 + 0.0 is r^2 bit for
+// bit), which costs a compare and a select per pair and keeps rsq(0) = inf
+// out of the arithmetic.
 template <int FM>
 __device__ __forceinline__ void pair_self(double px, double py, double pz, const double *s, bool self, double &ax,
                                           double &ay, double &az, Coef2 k) {
     const double dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+    const double one = self ? 1.0 : 0.0;
     if (FM == ABIE_FORCE_FP64_EXACT) {
-        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r2 = (dx * dx + dy * dy + dz * dz) + one;
         const double r3 = sqrt(r2) * r2;
-        const double tx = s[3] * dx / r3, ty = s[3] * dy / r3, tz = s[3] * dz / r3;
-        ax = ax - (self ? 0.0 : tx);
-        ay = ay - (self ? 0.0 : ty);
-        az = az - (self ? 0.0 : tz);
+        ax = ax - s[3] * dx / r3;
+        ay = ay - s[3] * dy / r3;
+        az = az - s[3] * dz / r3;
     } else {  // massive sinks always use FP64
-        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, one)));
         const double y = __builtin_amdgcn_rsq(r2);
         const double y2 = y * y;
         const double e = fma(-r2, y2, 1.0);
         const double g3 = (s[3] * y) * y2;
-        double t = fma(g3 * e, fma(e, k.c2, k.c1), g3);
-        t = self ? 0.0 : t;
+        const double t = fma(g3 * e, fma(e, k.c2, k.c1), g3);
         ax = fma(-t, dx, ax);
         ay = fma(-t, dy, ay);
         az = fma(-t, dz, az);
