@@ -772,46 +772,72 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 hit = any ? 1.0 : 0.0;
                 wg_max2<NT>(hit, dummy2, E.red);
                 if (hit > 0.0) {
-                    // exact pass in the reference's (j, k) order, one thread
-                    if (tid == 0) {
+                    // exact pass in the reference's (j, k) order by the first
+                    // wave: j in original order, the k > j in chunks of 64
+                    // lanes; the ring slots follow from ballots, so the rings
+                    // and counters end as the serial loop leaves them
+                    if (tid < 64) {
+                        const int lane = tid;
+                        const unsigned long long below = (1ull << lane) - 1ull;
                         long long nce = ctl->n_ce, nco = ctl->n_coll;
-                        double *rce = A.ring_ce ? A.ring_ce + (long long)sys * 4 * A.max_ce : nullptr;
-                        double *rco = A.ring_coll ? A.ring_coll + (long long)sys * 4 * A.max_coll : nullptr;
+                        double *rce = (A.ring_ce && A.max_ce > 0) ? A.ring_ce + (long long)sys * 4 * A.max_ce : nullptr;
+                        double *rco = (A.ring_coll && A.max_coll > 0) ? A.ring_coll + (long long)sys * 4 * A.max_coll
+                                                                       : nullptr;
                         for (int q1 = 0; q1 < n; ++q1) {
                             const int je = A.inv[b0 + q1];
                             const double jo = A.orig[b0 + je];
-                            for (int q2 = q1 + 1; q2 < n; ++q2) {
-                                const int ke = A.inv[b0 + q2];
-                                const double ko = A.orig[b0 + ke];
-                                const double dx = sx[b0 + je] - sx[b0 + ke], dyy = sy[b0 + je] - sy[b0 + ke],
-                                             dz = sz[b0 + je] - sz[b0 + ke];
-                                const double r2 = dx * dx + dyy * dyy + dz * dz;
-                                const double dd = sqrt(r2);
-                                const double rr = A.rad[b0 + je] + A.rad[b0 + ke];
-                                if (dd <= cd) {
-                                    if (rce && A.max_ce > 0) {
-                                        const long long w = 4LL * A.max_ce, o = 4 * nce;
-                                        rce[o % w] = t; rce[(o + 1) % w] = jo; rce[(o + 2) % w] = ko;
-                                        rce[(o + 3) % w] = dd;
-                                    }
-                                    nce += 1;
+                            const double xj = sx[b0 + je], yj = sy[b0 + je], zj = sz[b0 + je];
+                            const double rj = A.rad[b0 + je];
+                            for (int c = q1 + 1; c < n; c += 64) {
+                                const int q2 = c + lane;
+                                bool ce = false, co = false;
+                                double dd = 0.0, ko = 0.0;
+                                if (q2 < n) {
+                                    const int ke = A.inv[b0 + q2];
+                                    ko = A.orig[b0 + ke];
+                                    const double dx = xj - sx[b0 + ke], dyy = yj - sy[b0 + ke], dz = zj - sz[b0 + ke];
+                                    const double r2 = dx * dx + dyy * dyy + dz * dz;
+                                    dd = sqrt(r2);
+                                    const double rr = rj + A.rad[b0 + ke];
+                                    ce = dd <= cd;
+                                    co = rr > 0 && dd <= rr;
                                 }
-                                if (rr > 0 && dd <= rr) {
-                                    if (rco && A.max_coll > 0) {
-                                        const long long w = 4LL * A.max_coll, o = 4 * nco;
-                                        rco[o % w] = t; rco[(o + 1) % w] = jo; rco[(o + 2) % w] = ko;
-                                        rco[(o + 3) % w] = dd;
+                                const unsigned long long mce = __ballot(ce), mco = __ballot(co);
+                                if (mce | mco) {
+                                    const long long ece = nce + __popcll(mce), eco = nco + __popcll(mco);
+                                    // an event is written unless a later one of
+                                    // this chunk takes the same ring slot
+                                    if (ce && rce) {
+                                        const long long idx = nce + __popcll(mce & below);
+                                        if (idx >= ece - A.max_ce) {
+                                            const long long w = 4LL * A.max_ce, o = 4 * idx;
+                                            rce[o % w] = t; rce[(o + 1) % w] = jo; rce[(o + 2) % w] = ko;
+                                            rce[(o + 3) % w] = dd;
+                                        }
                                     }
-                                    nco += 1;
+                                    if (co && rco) {
+                                        const long long idx = nco + __popcll(mco & below);
+                                        if (idx >= eco - A.max_coll) {
+                                            const long long w = 4LL * A.max_coll, o = 4 * idx;
+                                            rco[o % w] = t; rco[(o + 1) % w] = jo; rco[(o + 2) % w] = ko;
+                                            rco[(o + 3) % w] = dd;
+                                        }
+                                    }
+                                    nce = ece;
+                                    nco = eco;
+                                    // the next chunk's writes may reuse these slots
+                                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                                 }
                             }
                         }
-                        ctl->n_ce = nce;
-                        ctl->n_coll = nco;
-                        int st = ST_RUNNING;
-                        if (A.max_ce > 0 && nce >= A.max_ce) st = ST_CE;
-                        else if (A.max_coll > 0 && nco >= A.max_coll) st = ST_COLL;
-                        flag[1] = (double)st;
+                        if (lane == 0) {
+                            ctl->n_ce = nce;
+                            ctl->n_coll = nco;
+                            int st = ST_RUNNING;
+                            if (A.max_ce > 0 && nce >= A.max_ce) st = ST_CE;
+                            else if (A.max_coll > 0 && nco >= A.max_coll) st = ST_COLL;
+                            flag[1] = (double)st;
+                        }
                     }
                     __syncthreads();
                     const int st = (int)flag[1];

@@ -308,3 +308,39 @@ def test_collision_scan_with_radii_across_tiles_matches_oracle(clib, orc, pair):
     assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
     coll = clib.get_collision_data().reshape(-1)
     assert same_bits(coll, s.buf_coll) and sorted(coll[1:3]) == [i, j]
+
+
+def test_many_events_in_one_step_wrap_rings_as_oracle(clib, orc):
+    """60 close test-particle pairs, 25 of them colliding, in one step: the
+    rings (7 and 3 slots) wrap many times inside one chunk of the parallel
+    exact pass; rings, counters and state as the oracle's GR15."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(911, 2, 400)
+    pos = pos.copy()
+    vel = vel.copy()
+    n = len(m)
+    r = np.zeros(n)
+    for q in range(60):
+        a, b = 2 + 5 * q, 4 + 5 * q
+        pos[b] = pos[a] + np.array([2e-4, 1e-4 * (q % 3), -1e-4])
+        vel[b] = vel[a]
+        if q % 12 < 5:
+            r[a] = r[b] = 2e-4
+    clib.initialize_code(G, 0.0, n, 7, 3, 1e-3, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, 0.5, 1.0)
+    except ParticleException as ex:
+        ret = 1 if "Close" in type(ex).__name__ else 2
+    clib.get_state(p, v, mm, rr)
+    s = orc.Sim(pos, vel, m, r, max_ce=7, max_coll=3, ce_dist=1e-3, cr_pow=True)
+    ref = s.integrate(0.5, 0.0)
+    assert ret == ref == 1
+    assert clib.get_model_time() == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
+    assert same_bits(clib.get_collision_data().reshape(-1), s.buf_coll)
