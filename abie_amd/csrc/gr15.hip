@@ -86,64 +86,115 @@ hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hip
     return hipErrorInvalidValue;
 }
 
-// check_collisions_close_encounters (common.c:116-158) for one system: all
-// threads filter the pairs with a test that can only over-report; if any
-// pair passes, thread 0 redoes the reference's ordered (j, k) loop exactly
-// and updates the counters (cnt[0] = close encounters, cnt[1] = collisions)
-// and the rings.  *ret receives 0, 1 or 2.
-__global__ __launch_bounds__(256) void events_kernel(const double *__restrict__ pos, const double *__restrict__ rad,
-                                                     int n, double t, double cd, long long *cnt, int max_ce,
-                                                     int max_coll, double *ring_ce, double *ring_coll, int *ret) {
-    __shared__ int hit;
-    if (threadIdx.x == 0) hit = 0;
-    __syncthreads();
+// check_collisions_close_encounters (common.c:116-158) for one system, in two
+// kernels.  events_filter_kernel: block b owns bodies j in [256 b, 256 b +
+// 256) and streams the bodies k >= 256 b through LDS tiles, testing each
+// unordered pair once with a test that can only over-report (r^2 in the
+// reference's operation order against cd^2 or (r_j + r_k)^2 with a 2^-50
+// slack for the rounding of sqrt); any pass sets *hit.  events_exact_kernel:
+// only if *hit, one wave redoes the reference's ordered (j, k) loop exactly,
+// j serially and the k > j in chunks of 64 lanes, each event's ring slot from
+// a ballot prefix count (an event is skipped when a later one of the same
+// chunk takes its slot), and updates the counters (cnt[0] = close encounters,
+// cnt[1] = collisions) and the rings as the serial loop would.
+constexpr int kEvTile = 256;
+
+__global__ __launch_bounds__(kEvTile) void events_filter_kernel(const double *__restrict__ pos,
+                                                                const double *__restrict__ rad, int n, double cd,
+                                                                int *hit) {
+    __shared__ double tx[kEvTile], ty[kEvTile], tz[kEvTile], tr[kEvTile];
+    const int j = blockIdx.x * kEvTile + threadIdx.x;
+    const bool valid = j < n;
+    const int jc = valid ? j : n - 1;
+    const double xj = pos[3 * jc], yj = pos[3 * jc + 1], zj = pos[3 * jc + 2], rj = rad[jc];
     const double cd2 = cd * cd * (1.0 + 0x1p-50);
-    int mine = 0;
-    for (int j = threadIdx.x; j < n; j += 256)
-        for (int k = j + 1; k < n; ++k) {
-            const double dx = pos[3 * j] - pos[3 * k], dy = pos[3 * j + 1] - pos[3 * k + 1],
-                         dz = pos[3 * j + 2] - pos[3 * k + 2];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double rr = rad[j] + rad[k];
-            mine |= (r2 <= cd2) || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50));
+    bool mine = false;
+    for (int k0 = blockIdx.x * kEvTile; k0 < n; k0 += kEvTile) {
+        const int cnt = min(kEvTile, n - k0);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            const int k = k0 + threadIdx.x;
+            tx[threadIdx.x] = pos[3 * k];
+            ty[threadIdx.x] = pos[3 * k + 1];
+            tz[threadIdx.x] = pos[3 * k + 2];
+            tr[threadIdx.x] = rad[k];
         }
-    if (mine) atomicOr(&hit, 1);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    long long nce = cnt[0], nco = cnt[1];
-    if (hit) {
-        for (int j = 0; j < n; ++j)
-            for (int k = j + 1; k < n; ++k) {
-                const double dx = pos[3 * j] - pos[3 * k], dy = pos[3 * j + 1] - pos[3 * k + 1],
-                             dz = pos[3 * j + 2] - pos[3 * k + 2];
-                const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                const double rr = rad[j] + rad[k];
-                if (d <= cd) {
-                    if (max_ce > 0) {
-                        const long long w = 4LL * max_ce, o = 4 * nce;
-                        ring_ce[o % w] = t; ring_ce[(o + 1) % w] = j; ring_ce[(o + 2) % w] = k; ring_ce[(o + 3) % w] = d;
-                    }
-                    nce += 1;
-                }
-                if (rr > 0 && d <= rr) {
-                    if (max_coll > 0) {
-                        const long long w = 4LL * max_coll, o = 4 * nco;
-                        ring_coll[o % w] = t; ring_coll[(o + 1) % w] = j; ring_coll[(o + 2) % w] = k;
-                        ring_coll[(o + 3) % w] = d;
-                    }
-                    nco += 1;
-                }
-            }
+        __syncthreads();
+        for (int q = 0; q < cnt; ++q) {
+            const double dx = xj - tx[q], dy = yj - ty[q], dz = zj - tz[q];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double rr = rj + tr[q];
+            mine |= (k0 + q > j) && ((r2 <= cd2) || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50)));
+        }
     }
-    cnt[0] = nce;
-    cnt[1] = nco;
-    *ret = (max_ce > 0 && nce >= max_ce) ? 1 : (max_coll > 0 && nco >= max_coll) ? 2 : 0;
+    if (valid && mine) atomicOr(hit, 1);
 }
 
+__global__ __launch_bounds__(64) void events_exact_kernel(const double *__restrict__ pos,
+                                                          const double *__restrict__ rad, int n, double t, double cd,
+                                                          long long *cnt, int max_ce, int max_coll, double *ring_ce,
+                                                          double *ring_coll, const int *hit, int *ret) {
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    long long nce = cnt[0], nco = cnt[1];
+    if (*hit) {
+        for (int j = 0; j < n; ++j) {
+            const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2], rj = rad[j];
+            for (int c = j + 1; c < n; c += 64) {
+                const int k = c + lane;
+                bool ce = false, co = false;
+                double d = 0.0;
+                if (k < n) {
+                    const double dx = xj - pos[3 * k], dy = yj - pos[3 * k + 1], dz = zj - pos[3 * k + 2];
+                    d = sqrt(dx * dx + dy * dy + dz * dz);
+                    const double rr = rj + rad[k];
+                    ce = d <= cd;
+                    co = rr > 0 && d <= rr;
+                }
+                const unsigned long long mce = __ballot(ce), mco = __ballot(co);
+                if (mce | mco) {
+                    const long long ece = nce + __popcll(mce), eco = nco + __popcll(mco);
+                    if (ce && max_ce > 0) {
+                        const long long idx = nce + __popcll(mce & below);
+                        if (idx >= ece - max_ce) {
+                            const long long w = 4LL * max_ce, o = 4 * idx;
+                            ring_ce[o % w] = t; ring_ce[(o + 1) % w] = j; ring_ce[(o + 2) % w] = k;
+                            ring_ce[(o + 3) % w] = d;
+                        }
+                    }
+                    if (co && max_coll > 0) {
+                        const long long idx = nco + __popcll(mco & below);
+                        if (idx >= eco - max_coll) {
+                            const long long w = 4LL * max_coll, o = 4 * idx;
+                            ring_coll[o % w] = t; ring_coll[(o + 1) % w] = j; ring_coll[(o + 2) % w] = k;
+                            ring_coll[(o + 3) % w] = d;
+                        }
+                    }
+                    nce = ece;
+                    nco = eco;
+                    // the next chunk's writes may reuse these slots
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        cnt[0] = nce;
+        cnt[1] = nco;
+        *ret = (max_ce > 0 && nce >= max_ce) ? 1 : (max_coll > 0 && nco >= max_coll) ? 2 : 0;
+    }
+}
+
+// flags[0] must be 0 on entry (hit flag); flags[1] receives 0, 1 or 2.
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
-                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *ret, hipStream_t st) {
-    hipLaunchKernelGGL(events_kernel, dim3(1), dim3(256), 0, st, pos, rad, n, t, cd, cnt, max_ce, max_coll, ring_ce,
-                       ring_coll, ret);
+                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, hipStream_t st) {
+    if (n <= 0) return hipErrorInvalidValue;
+    const int nb = (n + kEvTile - 1) / kEvTile;
+    hipLaunchKernelGGL(events_filter_kernel, dim3(nb), dim3(kEvTile), 0, st, pos, rad, n, cd, flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(events_exact_kernel, dim3(1), dim3(64), 0, st, pos, rad, n, t, cd, cnt, max_ce, max_coll,
+                       ring_ce, ring_coll, flags, flags + 1);
     return hipGetLastError();
 }
 

@@ -824,9 +824,11 @@ size_t check_collisions_close_encounters(const double *pos, const double *radii,
     long long cnt[2] = {(long long)g.n_ce, (long long)g.n_coll};
     int ret = 0;
     bool ok = dpos.reserve(3 * N) == hipSuccess && drad.reserve(N) == hipSuccess && dcnt.reserve(2) == hipSuccess &&
-              dret.reserve(1) == hipSuccess && dce.reserve(4 * std::max<size_t>(g.max_ce, 1)) == hipSuccess &&
+              dret.reserve(2) == hipSuccess && dce.reserve(4 * std::max<size_t>(g.max_ce, 1)) == hipSuccess &&
               dco.reserve(4 * std::max<size_t>(g.max_coll, 1)) == hipSuccess;
-    ok = ok && hipMemcpy(dpos.p, pos, 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+    const int flags0[2] = {0, 0};
+    ok = ok && hipMemcpy(dret.p, flags0, sizeof flags0, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dpos.p, pos, 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(drad.p, radii, N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(dcnt.p, cnt, sizeof cnt, hipMemcpyHostToDevice) == hipSuccess &&
          (g.max_ce == 0 || hipMemcpy(dce.p, g.buf_ce.data(), 4 * g.max_ce * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) &&
@@ -835,7 +837,7 @@ size_t check_collisions_close_encounters(const double *pos, const double *radii,
                              dret.p, g.stream) == hipSuccess &&
          hipStreamSynchronize(g.stream) == hipSuccess &&
          hipMemcpy(cnt, dcnt.p, sizeof cnt, hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(&ret, dret.p, sizeof ret, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(&ret, dret.p + 1, sizeof ret, hipMemcpyDeviceToHost) == hipSuccess &&
          (g.max_ce == 0 || hipMemcpy(g.buf_ce.data(), dce.p, 4 * g.max_ce * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) &&
          (g.max_coll == 0 || hipMemcpy(g.buf_coll.data(), dco.p, 4 * g.max_coll * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess);
     dpos.release(); drad.release(); dce.release(); dco.release(); dcnt.release(); dret.release();

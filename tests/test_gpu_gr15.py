@@ -344,3 +344,25 @@ def test_many_events_in_one_step_wrap_rings_as_oracle(clib, orc):
     assert same_bits(p, s.pos) and same_bits(v, s.vel)
     assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
     assert same_bits(clib.get_collision_data().reshape(-1), s.buf_coll)
+
+
+def test_check_events_many_bodies_matches_oracle(clib, orc):
+    """check_collisions_close_encounters on 1,500 bodies (six filter blocks),
+    50 close pairs, 20 of them touching, rings of 7 and 3 slots: return
+    codes, rings and counters over two calls as the oracle."""
+    rng = np.random.default_rng(12)
+    n = 1500
+    pos = rng.uniform(-10, 10, size=(n, 3))
+    r = np.zeros(n)
+    for q in range(50):
+        a, b = 7 + 29 * q, 20 + 29 * q
+        pos[b] = pos[a] + rng.normal(size=3) * 2e-3
+        if q % 5 < 2:
+            r[a] = r[b] = 5e-3
+    clib.initialize_code(G, 0.0, n, 7, 3, 1e-2, deviceID=0)
+    s = orc.Sim(pos, np.zeros_like(pos), np.ones(n), r, max_ce=7, max_coll=3, ce_dist=1e-2)
+    for t in (1.0, 2.0):
+        ret = clib.check_collisions_close_encounters(pos.reshape(-1), r, n, t)
+        assert ret == s.check_events(t=t)
+        assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
+        assert same_bits(clib.get_collision_data().reshape(-1), s.buf_coll)
