@@ -72,7 +72,7 @@ SIGNATURES = {
 }
 
 # constants of include/abie_hip.h
-FORCE_FP64, FORCE_FP64_EXACT, FORCE_MIXED = 0, 1, 2
+FORCE_FP64, FORCE_FP64_EXACT, FORCE_MIXED, FORCE_FP32 = 0, 1, 2, 3
 CTRL_PLAIN, CTRL_MASSIVE, CTRL_LEARNED = 0, 1, 2
 STEPLOG_FIELDS = 18
 MLP_IN, MLP_HIDDEN = 12, 64

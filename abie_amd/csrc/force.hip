@@ -99,7 +99,34 @@ struct Interact<ABIE_FORCE_MIXED> {
     }
 };
 
+// Test particle from a massive body in FP32 arithmetic (ABIE_FORCE_FP32):
+// FP64 relative coordinates rounded to FP32, v_rsq_f32 + one Newton step,
+// FP32 products, the term added to an FP64 sum (more accurate than a
+// float-float pair, and one add per component).  Kept to measure against the
+// mixed mode: without packed FP32 each FP32 op issues at the FP64 rate, and
+// the six conversions make it the slower of the two.
+template <>
+struct Interact<ABIE_FORCE_FP32> {
+    static __device__ __forceinline__ void run(double px, double py, double pz, double sx, double sy,
+                                               double sz, double gm, double &ax, double &ay, double &az, Coef) {
+        const float dx = (float)(px - sx), dy = (float)(py - sy), dz = (float)(pz - sz);
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        float y = __builtin_amdgcn_rsqf(r2);
+        const float hy = 0.5f * y;
+        const float e = fmaf(-r2 * y, y, 1.0f);
+        y = fmaf(hy, e, y);
+        const float t = ((float)gm * y) * (y * y);
+        ax = ax - (double)(t * dx);
+        ay = ay - (double)(t * dy);
+        az = az - (double)(t * dz);
+    }
+};
+
 constexpr int BLOCK = 256;
+
+// modes whose test-particle terms are reduced precision; their massive sinks
+// are evaluated in FP64 mode
+constexpr bool tp_reduced(int mode) { return mode == ABIE_FORCE_MIXED || mode == ABIE_FORCE_FP32; }
 
 // Final accelerations are written once and not read again by this launch:
 // streaming (non-temporal) stores keep them from piling up as dirty lines
@@ -172,8 +199,8 @@ __global__ __launch_bounds__(BLOCK) void accel_kernel(AccelArgs a, int j_chunk, 
         px[s] = a.px[ic]; py[s] = a.py[ic]; pz[s] = a.pz[ic];
         ax[s] = 0.0; ay[s] = 0.0; az[s] = 0.0;
     }
-    // MIXED: blocks holding any massive sink evaluate them all in FP64.
-    const bool block_fp64 = (MODE == ABIE_FORCE_MIXED) && (i0 < a.n_self);
+    // MIXED / FP32: blocks holding any massive sink evaluate them all in FP64.
+    const bool block_fp64 = tp_reduced(MODE) && (i0 < a.n_self);
     // sinks of this block that are also sources lie in [i0, i_self_end)
     const int i_self_end = min(i0 + BLOCK * SPT, a.n_self);
     const Coef kc = make_coef();
@@ -187,12 +214,12 @@ __global__ __launch_bounds__(BLOCK) void accel_kernel(AccelArgs a, int j_chunk, 
         }
         __syncthreads();
         const bool diag = (i0 < a.n_self) && (k0 < i_self_end) && (k0 + cnt > i0);
-        if (MODE == ABIE_FORCE_MIXED) {
+        if (tp_reduced(MODE)) {
             if (block_fp64) {
                 if (diag) tile_sum<SPT, ABIE_FORCE_FP64, true>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
                 else tile_sum<SPT, ABIE_FORCE_FP64, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
             } else {
-                tile_sum<SPT, ABIE_FORCE_MIXED, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
+                tile_sum<SPT, MODE, false>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
             }
         } else {
             if (diag) tile_sum<SPT, MODE, true>(tx, ty, tz, tg, cnt, k0, idx, px, py, pz, ax, ay, az, kc);
@@ -283,9 +310,9 @@ __global__ __launch_bounds__(BLOCK) void accel_scalar_kernel(AccelArgs a, int j_
     // [j0, j1) only when the ranges overlap: then skip k == i
     const int i_self_end = min(i0 + BLOCK * SPT, a.n_self);
     if (i0 < a.n_self && j0 < i_self_end && j1 > i0) {
-        constexpr int M = MODE == ABIE_FORCE_MIXED ? ABIE_FORCE_FP64 : MODE;
+        constexpr int M = tp_reduced(MODE) ? ABIE_FORCE_FP64 : MODE;
         scalar_sum<SPT, M, true>(a.sx, a.sy, a.sz, a.gm, j0, j1, idx, px, py, pz, ax, ay, az, kc);
-    } else if (MODE == ABIE_FORCE_MIXED && i0 < a.n_self) {
+    } else if (tp_reduced(MODE) && i0 < a.n_self) {
         scalar_sum<SPT, ABIE_FORCE_FP64, false>(a.sx, a.sy, a.sz, a.gm, j0, j1, idx, px, py, pz, ax, ay, az, kc);
     } else {
         scalar_sum<SPT, MODE, false>(a.sx, a.sy, a.sz, a.gm, j0, j1, idx, px, py, pz, ax, ay, az, kc);
@@ -398,6 +425,7 @@ hipError_t launch_accel(const AccelArgs &a, const AccelPlan &pl, double *part, h
         case ABIE_FORCE_FP64: return launch_spt<ABIE_FORCE_FP64>(a, pl, part, st);
         case ABIE_FORCE_FP64_EXACT: return launch_spt<ABIE_FORCE_FP64_EXACT>(a, pl, part, st);
         case ABIE_FORCE_MIXED: return launch_spt<ABIE_FORCE_MIXED>(a, pl, part, st);
+        case ABIE_FORCE_FP32: return launch_spt<ABIE_FORCE_FP32>(a, pl, part, st);
         default: return hipErrorInvalidValue;
     }
 }

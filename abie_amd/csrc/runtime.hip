@@ -886,7 +886,7 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz, con
                        double *ay, double *az, int mode, void *stream) {
     if (n_src < 0 || n_sink < 0 || n_self < 0 || n_self > n_src || n_self > n_sink)
         return fail(ABIE_HIP_EINVAL, "accel_soa: bad sizes (n_src %d, n_sink %d, n_self %d)", n_src, n_sink, n_self);
-    if (mode < 0 || mode > 2) return fail(ABIE_HIP_EINVAL, "accel_soa: bad mode %d", mode);
+    if (mode < 0 || mode > 3) return fail(ABIE_HIP_EINVAL, "accel_soa: bad mode %d", mode);
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     static thread_local DBuf<double> part;

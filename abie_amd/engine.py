@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib
 
-FORCE_MODES = {"fp64": _lib.FORCE_FP64, "exact": _lib.FORCE_FP64_EXACT, "mixed": _lib.FORCE_MIXED}
+FORCE_MODES = {"fp64": _lib.FORCE_FP64, "exact": _lib.FORCE_FP64_EXACT, "mixed": _lib.FORCE_MIXED,
+               "fp32": _lib.FORCE_FP32}  # fp32: force entry points only
 CONTROLLERS = {"plain": _lib.CTRL_PLAIN, "massive": _lib.CTRL_MASSIVE, "learned": _lib.CTRL_LEARNED}
 
 

@@ -283,14 +283,14 @@ def cpu_gr15_config0(years, chunk=None):
     return out
 
 
-def mixed_error_report(pos, m, Gc):
-    """Per-test-particle relative acceleration error of the mixed mode
-    against FP64 (massive sinks stay FP64)."""
+def mixed_error_report(pos, m, Gc, mode="mixed"):
+    """Per-test-particle relative acceleration error of a reduced-precision
+    mode ("mixed" or "fp32") against FP64 (massive sinks stay FP64)."""
     import torch
     from abie_amd import engine
     src, sinks, n_self, perm = engine.soa_from_bodies(pos, m, Gc)
     a64 = torch.stack(engine.accel_soa(src, sinks, n_self, "fp64"), 1)
-    amx = torch.stack(engine.accel_soa(src, sinks, n_self, "mixed"), 1)
+    amx = torch.stack(engine.accel_soa(src, sinks, n_self, mode), 1)
     tp = torch.as_tensor((m[perm] == 0), device=a64.device)
     rel = torch.linalg.norm(amx - a64, dim=1) / torch.linalg.norm(a64, dim=1)
     rel = rel[tp]
@@ -421,9 +421,13 @@ def main():
         # rsqrt seed), so its work is priced against the FP64 vector peak
         extras["config1a_mixed"] = {"interactions_per_s": n_int * args.steps / w, "mean_launch_us": ms * 1e3,
                                     "frac_fp64_peak": n_int * 20 / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        # the FP32 formulation of the test-particle terms, measured beside it
+        w, ms, _ = force_case(pos, m, G, "fp32", args.steps, args.warmup)
+        extras["config1a_fp32"] = {"interactions_per_s": n_int * args.steps / w, "mean_launch_us": ms * 1e3,
+                                   "frac_fp64_peak": n_int * 20 / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
         w, ms, _ = force_case(pos, m, G, "exact", reps, 3)
         extras["config1a_exact"] = {"interactions_per_s": n_int * reps / w, "mean_launch_us": ms * 1e3}
-        log("mixed/exact: %s %s" % (extras["config1a_mixed"], extras["config1a_exact"]))
+        log("mixed/fp32/exact: %s %s %s" % (extras["config1a_mixed"], extras["config1a_fp32"], extras["config1a_exact"]))
         dt = host_aos_case(pos, m)
         extras["config1a_host_aos_pcie"] = {"interactions_per_s": n_int / dt, "ms_per_call": dt * 1e3,
                                             "note": "calculate_accelerations on host AoS arrays: pageable "
@@ -434,7 +438,9 @@ def main():
         p2, _, m2 = ics.config0(n_test=100000)
         extras["mixed_error_config2"] = mixed_error_report(p2, m2, G)
         extras["mixed_error_config1a"] = mixed_error_report(pos, m, G)
-        log("mixed error: C2 %s, C1a %s" % (extras["mixed_error_config2"], extras["mixed_error_config1a"]))
+        extras["fp32_error_config2"] = mixed_error_report(p2, m2, G, "fp32")
+        extras["fp32_error_config1a"] = mixed_error_report(pos, m, G, "fp32")
+        log("mixed error: C2 %s, C1a %s; fp32 error: C2 %s, C1a %s" % (extras["mixed_error_config2"], extras["mixed_error_config1a"], extras["fp32_error_config2"], extras["fp32_error_config1a"]))
         extras["gr15_config0_cpu_reference_1yr"] = cpu_gr15_config0(1.0) if not args.no_cpu else None
         log("gr15 cpu reference: %s" % extras["gr15_config0_cpu_reference_1yr"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None

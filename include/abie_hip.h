@@ -127,6 +127,10 @@ size_t ode_n_body_first_order(double *pos, size_t N, double G, const double *mas
                                     single-precision accuracy (hardware rsqrt
                                     without correction, ~3*2^-24 relative);
                                     FP64 coordinates and accumulation            */
+#define ABIE_FORCE_FP32       3  /* massive->test in FP32 arithmetic: rsqrt +
+                                    one Newton step, FP64 accumulation (force
+                                    entry points abie_hip_accel_soa and
+                                    abie_hip_time_accel_soa only)               */
 
 /* Last error message of this thread's most recent failing call. */
 const char *abie_hip_last_error(void);

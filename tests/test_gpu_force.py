@@ -4,7 +4,8 @@ outputs (golden fixtures) and with the CPU oracle.
 Tolerances: ABIE_FORCE_FP64_EXACT is bit-identical; ABIE_FORCE_FP64 is within
 1e-14 relative per acceleration vector where the sum is not dominated by
 cancellation, and within 1e-14 of sum_j |a_ij| everywhere; MIXED test
-particles within 4e-7 relative (uncorrected hardware rsqrt), massive bodies as FP64."""
+particles within 4e-7 relative (uncorrected hardware rsqrt), FP32 test
+particles within 2e-6; massive bodies as FP64 in both."""
 import numpy as np
 import pytest
 
@@ -135,6 +136,12 @@ def test_mixed_mode_error_bounds(orc):
     assert np.array_equal(amx[:n_self], a64[:n_self])  # massive sinks stay FP64
     # v_rsq_f64 is documented to 2^-23 relative: r^-3 to 3 * 2^-23 = 3.6e-7
     assert rel.max() < 4e-7 and np.sqrt(np.mean(rel ** 2)) < 1e-7
+    # FP32 test-particle terms: coordinates rounded to FP32 (2^-24 relative
+    # to |d|) and an FP32 r^-3 after one Newton step
+    a32 = torch.stack(engine.accel_soa(src, sinks, n_self, "fp32"), 1).cpu().numpy()
+    rel32 = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    assert np.array_equal(a32[:n_self], a64[:n_self])
+    assert rel32.max() < 2e-6 and np.sqrt(np.mean(rel32 ** 2)) < 5e-7
 
 
 def test_calculate_accelerations_ext_acc(clib, orc):
@@ -163,7 +170,7 @@ def test_fp64_full_size_all_sinks_within_1e14_of_exact():
     assert float(rel.max()) <= 1e-14
 
 
-@pytest.mark.parametrize("mode", ["fp64", "exact", "mixed"])
+@pytest.mark.parametrize("mode", ["fp64", "exact", "mixed", "fp32"])
 @pytest.mark.parametrize("shape", [(64, 1 << 16), (300, 20000), (8, 999)])
 def test_kernel_forms_give_identical_bits(monkeypatch, mode, shape):
     """The scalar-source kernel and the LDS-tiled kernel (one sink per
