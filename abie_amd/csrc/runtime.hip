@@ -540,9 +540,20 @@ struct GlobalSim {
     DBuf<int> d_perm;
     hipStream_t stream = nullptr;
     int n_cu = 256;
+    // integrator_gr's engine, kept between calls while nothing it was built
+    // from (bodies, masses, radii, extra forces) changes: set_state,
+    // set_additional_forces and finalize_code drop it
+    abie_ensemble *gr_cache = nullptr;
 };
 
 GlobalSim g;
+
+void gr_cache_drop() {
+    if (g.gr_cache) {
+        abie_ensemble_destroy(g.gr_cache);
+        g.gr_cache = nullptr;
+    }
+}
 
 int check_device() {
     if (!g.inited) return fail(ABIE_HIP_ESTATE, "initialize_code has not been called");
@@ -586,25 +597,39 @@ int aos_accel(const double *pos, size_t N, double G, const double *m, double *ac
 }
 
 // GR15 of the global simulation through a one-system ensemble.
+// GR15 from (pos, vel) to t_end as one single-system ensemble.  cache: the
+// arrays are the global simulation's own and the engine is kept for the next
+// call; its device state then already holds these positions and velocities
+// (it wrote them back at the end of the previous call), so a later call only
+// resets the step controller -- every call restarts GR15 from a fresh initial
+// step, as the reference's does, and the trajectory is the same bits.
 int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t N, double G, double t,
-             double t_end, double *t_out) {
+             double t_end, double *t_out, bool cache) {
     if (N == 0) return fail(ABIE_HIP_EINVAL, "integrator_gr: no bodies (the reference never terminates here)");
     if (g.C > 0.0) return fail(ABIE_HIP_EUNSUP, "post-Newtonian forces (C > 0) are not part of the device path");
     int rc = check_device();
     if (rc) return rc;
-    abie_ensemble *e = new abie_ensemble();
-    e->dev = g.dev;
-    e->n_sys = 1;
-    e->G = G;
+    abie_ensemble *e = cache ? g.gr_cache : nullptr;
+    if (e == nullptr) {
+        e = new abie_ensemble();
+        e->dev = g.dev;
+        e->n_sys = 1;
+        e->G = G;
+        e->ctrl = ABIE_CTRL_PLAIN;
+        const int offs[2] = {0, (int)N};
+        const double *ext = (g.ext_on && g.ext.size() >= 3 * N) ? g.ext.data() : nullptr;
+        rc = ens_build(e, offs, pos, vel, m, r, ext);
+        if (rc == 0) rc = ens_rings(e, (int)g.max_ce, (int)g.max_coll);
+        if (rc != 0) {
+            abie_ensemble_destroy(e);
+            return rc;
+        }
+        if (cache) g.gr_cache = e;
+    }
     e->force_mode = g.force_mode;
-    e->ctrl = ABIE_CTRL_PLAIN;
     e->ce_dist = g.ce_dist;
     e->max_attempts = 4096;
     if (const char *s = getenv("ABIE_HIP_MAX_ATTEMPTS")) e->max_attempts = std::max(1, atoi(s));
-    const int offs[2] = {0, (int)N};
-    const double *ext = (g.ext_on && g.ext.size() >= 3 * N) ? g.ext.data() : nullptr;
-    rc = ens_build(e, offs, pos, vel, m, r, ext);
-    if (rc == 0) rc = ens_rings(e, (int)g.max_ce, (int)g.max_coll);
     // carry the event counters and rings of the global simulation
     if (rc == 0) {
         SysCtl c;
@@ -642,7 +667,8 @@ int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t 
             if (c.status < 0) rc = fail(ABIE_HIP_EHIP, "integrator_gr: device engine reported an error");
         }
     }
-    abie_ensemble_destroy(e);
+    if (!cache) abie_ensemble_destroy(e);
+    else if (rc != 0) gr_cache_drop();  // keep no engine in an unknown state
     return rc ? rc : ret;
 }
 
@@ -694,6 +720,7 @@ int initialize_code(double G, double C, size_t N_MAX, size_t MAX_N_CE, size_t MA
 }
 
 int finalize_code(void) {
+    gr_cache_drop();
     for (auto *b : {&g.d_pos, &g.d_m, &g.d_x, &g.d_y, &g.d_z, &g.d_gm, &g.d_ax, &g.d_ay, &g.d_az, &g.d_acc, &g.d_ext, &g.d_part})
         b->release();
     g.d_perm.release();
@@ -707,6 +734,7 @@ int finalize_code(void) {
 }
 
 void set_state(double *pos, double *vel, double *m, double *r, size_t N, double G, double C) {
+    gr_cache_drop();
     if (N > g.nmax) {
         g.nmax = N;
         g.pos.resize(3 * N); g.vel.resize(3 * N); g.m.resize(N); g.r.resize(N);
@@ -744,6 +772,7 @@ void reset_collision_buffer(void) {
 }
 
 size_t set_additional_forces(size_t N, double *ext_acc) {
+    gr_cache_drop();
     g.ext_on = true;
     if (g.ext.size() < 3 * N) g.ext.resize(3 * N, 0.0);
     std::copy(ext_acc, ext_acc + 3 * N, g.ext.begin());
@@ -783,7 +812,7 @@ int integrator_gr(double t, double t_end, double dt) {
     (void)dt;
     if (!g.inited) return fail(ABIE_HIP_ESTATE, "initialize_code has not been called");
     double t_out = t;
-    const int ret = run_gr15(g.pos.data(), g.vel.data(), g.m.data(), g.r.data(), g.n, g.G, t, t_end, &t_out);
+    const int ret = run_gr15(g.pos.data(), g.vel.data(), g.m.data(), g.r.data(), g.n, g.G, t, t_end, &t_out, true);
     if (ret >= 0) g.t = t_out;
     return ret;
 }
@@ -792,7 +821,7 @@ size_t integrator_gauss_radau15(double *pos, double *vel, double *m, double *r, 
                                 double t_end, double dt) {
     (void)dt;
     double t_out = t;
-    const int ret = run_gr15(pos, vel, m, r, N, G, t, t_end, &t_out);
+    const int ret = run_gr15(pos, vel, m, r, N, G, t, t_end, &t_out, false);
     if (ret >= 0) g.t = t_out;
     return (size_t)(long long)ret;
 }

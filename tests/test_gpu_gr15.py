@@ -366,3 +366,42 @@ def test_check_events_many_bodies_matches_oracle(clib, orc):
         assert ret == s.check_events(t=t)
         assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
         assert same_bits(clib.get_collision_data().reshape(-1), s.buf_coll)
+
+
+def test_integrator_gr_reuses_engine_until_state_changes(clib, orc):
+    """integrator_gr keeps its engine between calls; set_state (new bodies)
+    and set_additional_forces drop it: each run matches the oracle started
+    from the state the caller last set (exact forces, bit for bit)."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(77, 4, 30)
+    n = len(m)
+    r = np.zeros(n)
+    clib.initialize_code(G, 0.0, n, 1, 1, 0.0, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v = pos.reshape(-1).copy(), vel.reshape(-1).copy()
+    clib.set_state(p, v, m.copy(), r.copy(), n, G, 0.0)
+    s = orc.Sim(pos, vel, m, cr_pow=True)
+    for t0, t1 in ((0.0, 0.2), (0.2, 0.4)):  # second call on the kept engine
+        clib.integrator_gr(clib.get_model_time() if t0 else 0.0, t1, 1.0)
+        s.integrate(t1, s.t if t0 else 0.0)
+        clib.get_state(p, v, m.copy(), r.copy())
+        assert clib.get_model_time() == s.t
+        assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    # new bodies: the engine must start from them
+    pos2, vel2 = pos * 1.01, vel * 0.99
+    clib.set_state(pos2.reshape(-1).copy(), vel2.reshape(-1).copy(), m.copy(), r.copy(), n, G, 0.0)
+    s2 = orc.Sim(pos2, vel2, m, cr_pow=True)
+    clib.integrator_gr(0.0, 0.3, 1.0)
+    s2.integrate(0.3, 0.0)
+    clib.get_state(p, v, m.copy(), r.copy())
+    assert same_bits(p, s2.pos) and same_bits(v, s2.vel)
+    # extra forces switched on: the engine must see them
+    ext = np.random.default_rng(3).normal(size=3 * n) * 1e-3
+    clib.set_additional_forces(ext)
+    s2 = orc.Sim(p.reshape(-1, 3), v.reshape(-1, 3), m, ext_acc=ext, cr_pow=True)
+    t = clib.get_model_time()
+    clib.integrator_gr(t, t + 0.2, 1.0)
+    s2.integrate(t + 0.2, t)
+    clib.get_state(p, v, m.copy(), r.copy())
+    assert same_bits(p, s2.pos) and same_bits(v, s2.vel)
+    clib.set_additional_forces(np.zeros(3 * n))
