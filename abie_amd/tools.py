@@ -46,6 +46,54 @@ class Tools(object):
     """Reference-style names (abie/tools.py)."""
 
     @staticmethod
+    def compute_energy(sol_state, masses, const_g):
+        """Total energy of every row of ``sol_state`` = [positions (3N) |
+        velocities (3N)] (abie/tools.py:5-33), vectorised over rows; None for
+        a state whose width is not a multiple of 6 (as the reference)."""
+        sol_state = np.atleast_2d(np.asarray(sol_state, dtype=np.float64))
+        n_rows, n_cols = sol_state.shape
+        if n_cols % 6 != 0 or n_rows == 0:
+            return None
+        n = n_cols // 6
+        m = np.asarray(masses, dtype=np.float64)[:n]
+        pos = sol_state[:, :3 * n].reshape(n_rows, n, 3)
+        vel = sol_state[:, 3 * n:].reshape(n_rows, n, 3)
+        ke = 0.5 * np.sum(m * np.sum(vel * vel, axis=2), axis=1)
+        d = np.linalg.norm(pos[:, :, None, :] - pos[:, None, :, :], axis=3)
+        with np.errstate(divide="ignore"):
+            inv = np.where(np.eye(n, dtype=bool), 0.0, 1.0 / d)
+        return ke - 0.5 * const_g * np.einsum("i,j,tij->t", m, m, inv)
+
+    @staticmethod
+    def from_cartesian_to_aei(mp, ms, position, velocity, G=0.0172020989 ** 2, z_vector=np.array([0.0, 0.0, 1.0])):
+        """Semi-major axis, eccentricity and inclination (abie/tools.py:235-318),
+        with the reference's default G (the Gaussian constant squared: au, day)
+        and its layouts: one body as 3-vectors, or many as (3, n) arrays (one
+        column per body); a body at r = 0 gives NaNs."""
+        position = np.asarray(position, dtype=np.float64)
+        velocity = np.asarray(velocity, dtype=np.float64)
+        mu = G * (mp + ms)
+        if position.ndim == 1 and velocity.ndim == 1:
+            r = np.sqrt(np.dot(position, position))
+            if r == 0:
+                return np.nan, np.nan, np.nan
+            pos, vel = position[:, None], velocity[:, None]
+        else:
+            pos, vel = position, velocity
+        v2 = np.sum(vel * vel, axis=0)
+        r = np.sqrt(np.sum(pos * pos, axis=0))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            a = -mu / (2.0 * (0.5 * v2 - mu / r))
+            h = np.cross(pos.T, vel.T).T
+            hunit = h / np.sqrt(np.sum(h * h, axis=0))
+            evec = ((v2 - mu / r) * pos - np.sum(pos * vel, axis=0) * vel) / mu
+            e = np.linalg.norm(evec, axis=0)
+            inc = np.arccos(np.matmul(z_vector, hunit))
+        if position.ndim == 1 and velocity.ndim == 1:
+            return float(a[0]), float(e[0]), float(inc[0])
+        return a, e, inc
+
+    @staticmethod
     def from_orbital_elements_to_cartesian(mp, ms, semimajor_axis, eccentricity, true_anomaly, inclination,
                                            argument_of_periapsis, longitude_of_ascending_node, G=G_SOLAR):
         p, v = elements_to_cartesian(mp, ms, semimajor_axis, eccentricity, inclination,

@@ -51,3 +51,32 @@ def test_plugins_keep_reference_names():
     from abie_amd.integrator import Integrator
     names = set(Integrator.load_integrators())
     assert names >= {"GaussRadau15", "WisdomHolman", "RungeKutta", "AdamsBashforth", "LeapFrog", "Euler"}
+
+
+def test_tools_energy_and_aei_closed_forms():
+    """Tools.compute_energy on a two-body circular orbit (E = -G m1 m2 / 2a)
+    and Tools.from_cartesian_to_aei on known orbits, one body and (3, n)."""
+    from abie_amd.tools import Tools
+    G = 4 * np.pi ** 2
+    m1, m2, a = 1.0, 1e-3, 2.0
+    v = np.sqrt(G * (m1 + m2) / a)
+    pos = np.array([[-a * m2 / (m1 + m2), 0, 0], [a * m1 / (m1 + m2), 0, 0]])
+    vel = np.array([[0, -v * m2 / (m1 + m2), 0], [0, v * m1 / (m1 + m2), 0]])
+    row = np.concatenate([pos.reshape(-1), vel.reshape(-1)])
+    e = Tools.compute_energy(np.stack([row, row]), [m1, m2], G)
+    assert np.allclose(e, -G * m1 * m2 / (2 * a), rtol=1e-13, atol=0)
+    assert Tools.compute_energy(np.zeros((2, 5)), [1.0], G) is None
+    # circular orbit of radius 3 inclined by 0.4 rad, and an eccentric one
+    r0, inc = 3.0, 0.4
+    vc = np.sqrt(G / r0)
+    p1 = np.array([r0, 0.0, 0.0])
+    v1 = np.array([0.0, vc * np.cos(inc), vc * np.sin(inc)])
+    aa, ee, ii = Tools.from_cartesian_to_aei(0.0, 1.0, p1, v1, G=G)
+    assert abs(aa - r0) < 1e-12 and ee < 1e-12 and abs(ii - inc) < 1e-12
+    ecc, ap = 0.3, 2.0
+    p2 = np.array([0.0, ap * (1 - ecc), 0.0])
+    v2 = np.array([-np.sqrt(G / ap * (1 + ecc) / (1 - ecc)), 0.0, 0.0])
+    A, E, I = Tools.from_cartesian_to_aei(0.0, 1.0, np.stack([p1, p2], 1), np.stack([v1, v2], 1), G=G)
+    assert np.allclose(A, [r0, ap], rtol=1e-12) and np.allclose(E, [0, ecc], atol=1e-12)
+    assert np.allclose(I, [inc, 0.0], atol=1e-12)
+    assert all(np.isnan(Tools.from_cartesian_to_aei(0.0, 1.0, np.zeros(3), v1, G=G)))
