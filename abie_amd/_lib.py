@@ -45,6 +45,8 @@ SIGNATURES = {
     "gpu_finalize": (None, []),
     "integrator_rk": (c_i, [c_d, c_d, c_d]),
     "integrator_wh": (c_i, [c_d, c_d, c_d]),
+    "integrator_runge_kutta": (None, [c_dp, c_dp, c_dp, c_dp, c_sz, c_d, c_d, c_d, c_d]),
+    "integrator_wisdom_holman": (None, [c_dp, c_dp, c_dp, c_dp, c_sz, c_d, c_d, c_d, c_d]),
     "ode_n_body_first_order": (c_sz, [c_dp, c_sz, c_d, c_dp, c_dp]),
     # device entry points
     "abie_hip_last_error": (ctypes.c_char_p, []),

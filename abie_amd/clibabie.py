@@ -102,6 +102,27 @@ class CLibABIE(object):
     def integrator_wh(self, t, t_end, dt):
         return _lib.check(self.lib.integrator_wh(t, t_end, dt), "integrator_wh")
 
+    def integrator_runge_kutta(self, pos, vel, masses, N, G, t, t_end, dt):
+        """abie/clibabie.py:112-122: not part of the device path; the library
+        only records the error, raised here instead of returning silently."""
+        r = np.zeros(int(N))
+        self.lib.integrator_runge_kutta(_dp(pos), _dp(vel), _dp(_f64(masses)), _dp(r), int(N), G, t, t_end, dt)
+        raise NotImplementedError(self.lib.abie_hip_last_error().decode())
+
+    def integrator_wisdom_holman(self, pos, vel, masses, N, G, t, t_end, dt):
+        """abie/clibabie.py:136-146: as integrator_runge_kutta."""
+        r = np.zeros(int(N))
+        self.lib.integrator_wisdom_holman(_dp(pos), _dp(vel), _dp(_f64(masses)), _dp(r), int(N), G, t, t_end, dt)
+        raise NotImplementedError(self.lib.abie_hip_last_error().decode())
+
+    def ode_n_body_first_order(self, x, N, G, masses):
+        """abie/clibabie.py:174-183: not part of the device path."""
+        dxdt = np.zeros(np.asarray(x).size)
+        rc = ctypes.c_long(self.lib.ode_n_body_first_order(_dp(_f64(x)), int(N), G, _dp(_f64(masses)),
+                                                           _dp(dxdt))).value
+        _lib.check(rc, "ode_n_body_first_order")
+        return dxdt
+
     def ode_n_body_second_order(self, x, N, G, masses):
         x = _f64(x)
         acc = np.zeros(x.size, dtype=np.double)

@@ -905,6 +905,12 @@ void gpu_finalize(void) { finalize_code(); }
 
 int integrator_rk(double, double, double) { return fail(ABIE_HIP_EUNSUP, "RungeKutta is outside the device path"); }
 int integrator_wh(double, double, double) { return fail(ABIE_HIP_EUNSUP, "WisdomHolman is outside the device path"); }
+void integrator_runge_kutta(double *, double *, double *, double *, size_t, double, double, double, double) {
+    fail(ABIE_HIP_EUNSUP, "RungeKutta is outside the device path");
+}
+void integrator_wisdom_holman(double *, double *, double *, double *, size_t, double, double, double, double) {
+    fail(ABIE_HIP_EUNSUP, "WisdomHolman is outside the device path");
+}
 size_t ode_n_body_first_order(double *, size_t, double, const double *, double *) {
     return (size_t)(long long)fail(ABIE_HIP_EUNSUP, "ode_n_body_first_order is outside the device path");
 }

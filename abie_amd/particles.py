@@ -158,6 +158,14 @@ class Particles(object):
         self._n += 1
         return k
 
+    def add_particle(self, particle):
+        """Add a Particle object (abie/particles.py:181-198); anything else is
+        a TypeError, as in the reference."""
+        if not isinstance(particle, Particle):
+            raise TypeError("Incompatible particle type.")
+        return self.add(pos=particle.pos, vel=particle.vel, mass=particle.mass, name=particle.name,
+                        radius=particle.radius, ptype=particle.ptype, primary=particle.primary)
+
     def add_many(self, pos, vel, masses, radii=None):
         """Bulk add of Cartesian states (n, 3) -- O(n)."""
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)

@@ -115,6 +115,12 @@ void gpu_finalize(void);
  * lookups succeed; they return ABIE_HIP_EUNSUP and change nothing. */
 int integrator_rk(double t, double t_end, double dt);
 int integrator_wh(double t, double t_end, double dt);
+/* src/common.h:119,121: void in the reference as well; they only record
+ * ABIE_HIP_EUNSUP as the last error (abie_hip_last_error). */
+void integrator_runge_kutta(double *pos, double *vel, double *m_vec, double *r_vec, size_t N, double G,
+                            double t, double t_end, double dt);
+void integrator_wisdom_holman(double *pos, double *vel, double *m_vec, double *r_vec, size_t N, double G,
+                              double t, double t_end, double dt);
 size_t ode_n_body_first_order(double *pos, size_t N, double G, const double *masses, double *dxdt);
 
 /* ======================= part 2: device entry points ==================== */

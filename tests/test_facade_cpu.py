@@ -80,3 +80,26 @@ def test_tools_energy_and_aei_closed_forms():
     assert np.allclose(A, [r0, ap], rtol=1e-12) and np.allclose(E, [0, ecc], atol=1e-12)
     assert np.allclose(I, [inc, 0.0], atol=1e-12)
     assert all(np.isnan(Tools.from_cartesian_to_aei(0.0, 1.0, np.zeros(3), v1, G=G)))
+
+
+def test_particle_module_and_add_particle():
+    from abie_amd.particle import Particle
+    from abie_amd.particles import Particles
+    ps = Particles(4 * np.pi ** 2)
+    ps.add(mass=1.0, name="Sun")
+    ps.add_particle(Particle(mass=1e-3, pos=np.array([5.2, 0.0, 0.0]), vel=np.array([0.0, 2.75, 0.0]),
+                             name="Jupiter", radius=4.7e-4))
+    assert ps.N == 2 and ps.masses[1] == 1e-3 and ps.radii[1] == 4.7e-4
+    assert np.array_equal(ps.positions.reshape(-1, 3)[1], [5.2, 0.0, 0.0])
+    with pytest.raises(TypeError):
+        ps.add_particle("not a particle")
+
+
+def test_out_of_path_integrators_raise_clearly():
+    from abie_amd.clibabie import CLibABIE
+    c = CLibABIE()
+    x = np.zeros(6)
+    with pytest.raises(NotImplementedError, match="RungeKutta"):
+        c.integrator_runge_kutta(x.copy(), x.copy(), np.ones(2), 2, 1.0, 0.0, 1.0, 0.1)
+    with pytest.raises(NotImplementedError, match="WisdomHolman"):
+        c.integrator_wisdom_holman(x.copy(), x.copy(), np.ones(2), 2, 1.0, 0.0, 1.0, 0.1)
