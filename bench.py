@@ -297,16 +297,22 @@ def mixed_error_report(pos, m, Gc, mode="mixed"):
     return {"n_test": int(tp.sum()), "max_rel_err": float(rel.max()), "rms_rel_err": float(torch.sqrt((rel ** 2).mean()))}
 
 
-def mixed_long_run(n_test, years, max_seconds=300.0):
+def mixed_long_run(n_test, years, max_seconds=300.0, proposer=None):
     """GR15 of the 1 star + 4 planets system with n_test test particles under
     the massive-only controller (test particles sliced over workgroups), FP64
-    vs mixed forces: |dE/E| and the test particles' final separation."""
+    vs mixed forces: |dE/E| and the test particles' final separation; with a
+    proposer also the learned controller on mixed forces."""
     from abie_amd import engine, ics
     pos, vel, m = ics.config0(n_test=n_test)
     out = {}
     fin = {}
-    for fm in ("fp64", "mixed"):
-        ens = engine.Ensemble([(pos, vel, m)], G=G, force_mode=fm, controller="massive")
+    runs = [("fp64", "fp64", "massive"), ("mixed", "mixed", "massive")]
+    if proposer is not None:
+        runs.append(("mixed/learned", "mixed", "learned"))
+    for fm_name, fm, ctrl in runs:
+        ens = engine.Ensemble([(pos, vel, m)], G=G, force_mode=fm, controller=ctrl)
+        if ctrl == "learned":
+            ens.set_proposer(proposer)
         e0 = ens.energy()
         t0 = time.perf_counter()
         done = ens.run(years, max_seconds=max_seconds)
@@ -315,8 +321,8 @@ def mixed_long_run(n_test, years, max_seconds=300.0):
         e1 = ens.energy()
         c = ens.counters()
         ens.close()
-        fin[fm] = (p, t[0])
-        out[fm] = {"done": bool(done), "model_yr": float(t[0]), "wall_s": wall, "dE_E": float(abs((e1[0] - e0[0]) / e0[0])),
+        fin[fm_name] = (p, t[0])
+        out[fm_name] = {"done": bool(done), "model_yr": float(t[0]), "wall_s": wall, "dE_E": float(abs((e1[0] - e0[0]) / e0[0])),
                    "evals": int(c[0, 0]), "system_yr_per_s": float(t[0] / wall)}
     if fin["fp64"][1] == fin["mixed"][1]:
         d = np.linalg.norm(fin["mixed"][0][5:] - fin["fp64"][0][5:], axis=1) / np.linalg.norm(fin["fp64"][0][5:], axis=1)
@@ -448,9 +454,9 @@ def main():
     if args.configs:
         extras["gr15_dropin_config0_100yr"] = gr15_dropin(100.0)
         log("config 0, 100 yr: %s" % extras["gr15_dropin_config0_100yr"])
-        extras["mixed_run_config2"] = mixed_long_run(100000, 1000.0)
-        log("config 2 run: %s" % extras["mixed_run_config2"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
+        extras["mixed_run_config2"] = mixed_long_run(100000, 1000.0, proposer=w)
+        log("config 2 run: %s" % extras["mixed_run_config2"])
         extras["ensemble_config3_4096x1yr"] = ensemble_report(4096, 1.0, w)
         if not args.no_cpu:
             extras["cpu_reference_plummer65536"] = cpu_plummer()
