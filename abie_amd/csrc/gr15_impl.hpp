@@ -416,6 +416,7 @@ __device__ __forceinline__ bool scan_tile(const double *tile, int q0, int cnt, i
 template <int NT, int BPT, int FM, int CT, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void gr15_kernel(GR15Args A) {
     extern __shared__ double lds[];
+    const long long t_enter = wall_clock64();
     const int sys = A.order[blockIdx.x];
     const SysDesc d = A.sys[sys];
     SysCtl *ctl = A.ctl + sys;
@@ -520,6 +521,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
     // ---- the step loop (integrator_gauss_radau15.c:279-368)
     double dy[BPT][3];
     while (n_att < A.max_attempts) {
+        // end the launch once its wall-clock budget is spent (the host then
+        // relaunches; the state is stored below either way), so that no
+        // unit waits on a fixed attempt count of the slowest one
+        if (A.budget_ticks > 0 && n_att > 0 && (n_att & 15) == 0) {
+            if (tid == 0) flag[2] = (wall_clock64() - t_enter > A.budget_ticks) ? 1.0 : 0.0;
+            __syncthreads();
+            if (flag[2] != 0.0) break;
+        }
         int pc_used = 0;
         double pc_last = DBL_MAX;
         for (int ipc = 0; ipc < 12; ++ipc) {

@@ -96,7 +96,9 @@ struct abie_ensemble {
     int events = 1;
     double ce_dist = 0.0;
     int max_ce = 0, max_coll = 0;
-    int max_attempts = 256;
+    int max_attempts = 1 << 30;       // per launch; the wall-clock budget ends launches
+    double launch_seconds = 2.0;      // wall-clock budget of one launch
+    long long clock_khz = 0;          // device wall-clock rate
     int n_units = 0;                   // work units (a system, or a slice of one)
     std::vector<int> offsets;          // [n_sys+1] original layout
     std::vector<long long> eng2orig;   // engine index -> original global index (-1: replica)
@@ -326,6 +328,7 @@ static GR15Args ens_args(abie_ensemble *e) {
     a.mlp = e->has_mlp ? e->d_mlp.p : nullptr;
     a.mratio = e->d_mratio.p;
     a.max_attempts = e->max_attempts;
+    a.budget_ticks = (long long)(e->launch_seconds * 1e3 * (double)e->clock_khz);
     a.events = e->events;
     return a;
 }
@@ -348,6 +351,12 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     std::vector<SysCtl> ctl(e->n_units);
     int done = 0;
     HIP_TRY(hipSetDevice(e->dev));
+    if (e->clock_khz == 0) {
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->dev));
+        e->clock_khz = khz;
+    }
+    if (const char *s = getenv("ABIE_HIP_LAUNCH_SECONDS")) e->launch_seconds = atof(s);
     for (;;) {
         GR15Args a = ens_args(e);
         for (int c = 0; c < kNumCfg; ++c) {
@@ -628,7 +637,7 @@ int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t 
     }
     e->force_mode = g.force_mode;
     e->ce_dist = g.ce_dist;
-    e->max_attempts = 4096;
+    e->max_attempts = 1 << 30;
     if (const char *s = getenv("ABIE_HIP_MAX_ATTEMPTS")) e->max_attempts = std::max(1, atoi(s));
     // carry the event counters and rings of the global simulation
     if (rc == 0) {
