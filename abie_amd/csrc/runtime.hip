@@ -329,6 +329,8 @@ static GR15Args ens_args(abie_ensemble *e) {
     a.mratio = e->d_mratio.p;
     a.max_attempts = e->max_attempts;
     a.budget_ticks = (long long)(e->launch_seconds * 1e3 * (double)e->clock_khz);
+    a.prefetch = 1;
+    if (const char *s = getenv("ABIE_HIP_PREFETCH")) a.prefetch = atoi(s) != 0;  // same bits either way
     a.events = e->events;
     return a;
 }

@@ -89,3 +89,28 @@ def test_skipped_event_scan_changes_nothing(monkeypatch):
     pa, va, ta, _ = a.state()
     pb, vb, tb, _ = b.state()
     assert np.array_equal(ta, tb) and same_bits(pa, pb) and same_bits(va, vb)
+
+
+@pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "massive"), ("fp64", "learned")])
+def test_prefetching_source_loop_same_bits(monkeypatch, force_mode, controller):
+    """Units with 24 or more massive bodies sum their sources with the
+    prefetching loop under the massive-only controllers; switched off, the
+    plain loop gives the same bits (odd and even source counts)."""
+    from abie_amd import engine, ics
+    systems = [ics.planetary_system(800 + k, nm, nt) for k, (nm, nt) in
+               enumerate([(24, 40), (31, 200), (41, 0), (64, 300), (23, 50)])]
+    w = None
+    if controller == "learned":
+        import os
+        w = np.load(os.path.join(os.path.dirname(__file__), "..", "abie_amd", "data", "proposer.npz"))["weights"]
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("ABIE_HIP_PREFETCH", flag)
+        e = engine.Ensemble(systems, force_mode=force_mode, controller=controller)
+        if w is not None:
+            e.set_proposer(w)
+        e.run(0.2)
+        out.append(e.state())
+        e.close()
+    (pa, va, ta, _), (pb, vb, tb, _) = out
+    assert np.array_equal(ta, tb) and same_bits(pa, pb) and same_bits(va, vb)
