@@ -66,7 +66,8 @@ struct GR15Args {
     const double *mratio;  // per unit: (sum of the other massive masses) / largest mass
     int max_attempts;  // per launch
     long long budget_ticks;  // wall-clock ticks a launch may run (checked every 16 attempts); 0: no limit
-    int prefetch;            // 1: units with many massive bodies use the prefetching source loop
+    int prefetch;            // units with at least this many massive bodies use the prefetching
+                             // source loop (massive-only controllers); 0: never
     int events;        // 0: skip the close-encounter / collision scan
 };
 

@@ -46,9 +46,6 @@ constexpr double kEpsb = 1.e-4, kFac = 0.25, kTolpc = 1.e-16, kHmin = 1.e-13;
 // plain controller's dtreq, so each step is at most as long as the plain
 // rule's and the proposer only removes rejected attempts.
 constexpr double kPropLo = -0.3, kPropHi = 0.0;
-// units with at least this many massive bodies use the prefetching source
-// loop (massive-only controllers)
-constexpr int kPrefetchNm = 24;
 
 // Every FP64 constant of the step is read from this table through a pointer
 // the compiler cannot see through (launder()), so each use is a scalar load
@@ -383,7 +380,7 @@ struct Engine {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < BPT; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
-        if (CT != ABIE_CTRL_PLAIN && A.prefetch && nm >= kPrefetchNm) {
+        if (CT != ABIE_CTRL_PLAIN && A.prefetch > 0 && nm >= A.prefetch) {
             sum_prefetched(S, p, acc);
         } else if (wave_massive) {
             // slot 0 holds massive bodies (nm <= NT): FP64 with the self-pair removed

@@ -93,18 +93,18 @@ def test_skipped_event_scan_changes_nothing(monkeypatch):
 
 @pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "massive"), ("fp64", "learned")])
 def test_prefetching_source_loop_same_bits(monkeypatch, force_mode, controller):
-    """Units with 24 or more massive bodies sum their sources with the
-    prefetching loop under the massive-only controllers; switched off, the
-    plain loop gives the same bits (odd and even source counts)."""
+    """Units with many massive bodies sum their sources with the prefetching
+    loop under the massive-only controllers; with it on for every unit and
+    off for all, the same bits (odd and even source counts)."""
     from abie_amd import engine, ics
     systems = [ics.planetary_system(800 + k, nm, nt) for k, (nm, nt) in
-               enumerate([(24, 40), (31, 200), (41, 0), (64, 300), (23, 50)])]
+               enumerate([(2, 40), (3, 10), (24, 40), (31, 200), (41, 0), (64, 300), (23, 50)])]
     w = None
     if controller == "learned":
         import os
         w = np.load(os.path.join(os.path.dirname(__file__), "..", "abie_amd", "data", "proposer.npz"))["weights"]
     out = []
-    for flag in ("1", "0"):
+    for flag in ("2", "0"):  # from 2 massive bodies on, and never
         monkeypatch.setenv("ABIE_HIP_PREFETCH", flag)
         e = engine.Ensemble(systems, force_mode=force_mode, controller=controller)
         if w is not None:
