@@ -55,12 +55,12 @@ size_t gr15_lds_bytes(int cfg) { return (size_t)2 * src_cap(kCfgs[cfg].nt) * 4 *
 
 
 // one translation unit per (force mode, controller) pair, compiled in parallel
-hipError_t launch_gr15_fp64_plain(int, const GR15Args &, int, size_t, hipStream_t);
-hipError_t launch_gr15_fp64_massive(int, const GR15Args &, int, size_t, hipStream_t);
-hipError_t launch_gr15_fp64_learned(int, const GR15Args &, int, size_t, hipStream_t);
-hipError_t launch_gr15_exact_plain(int, const GR15Args &, int, size_t, hipStream_t);
-hipError_t launch_gr15_mixed_massive(int, const GR15Args &, int, size_t, hipStream_t);
-hipError_t launch_gr15_mixed_learned(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15_fp64_plain(int, const GR15Args &, int, size_t, hipStream_t, bool);
+hipError_t launch_gr15_fp64_massive(int, const GR15Args &, int, size_t, hipStream_t, bool);
+hipError_t launch_gr15_fp64_learned(int, const GR15Args &, int, size_t, hipStream_t, bool);
+hipError_t launch_gr15_exact_plain(int, const GR15Args &, int, size_t, hipStream_t, bool);
+hipError_t launch_gr15_mixed_massive(int, const GR15Args &, int, size_t, hipStream_t, bool);
+hipError_t launch_gr15_mixed_learned(int, const GR15Args &, int, size_t, hipStream_t, bool);
 
 bool gr15_combo_supported(int fm, int ctrl) {
     if (fm == ABIE_FORCE_FP64) return ctrl >= 0 && ctrl <= 2;
@@ -69,19 +69,19 @@ bool gr15_combo_supported(int fm, int ctrl) {
     return false;
 }
 
-hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st) {
+hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st, bool coop) {
     if (nb <= 0) return hipSuccess;
     if (cfg < 0 || cfg >= kNumCfg) return hipErrorInvalidValue;
     const size_t lds = gr15_lds_bytes(cfg);
     if (fm == ABIE_FORCE_FP64) {
-        if (ctrl == ABIE_CTRL_PLAIN) return launch_gr15_fp64_plain(cfg, a, nb, lds, st);
-        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_fp64_massive(cfg, a, nb, lds, st);
-        return launch_gr15_fp64_learned(cfg, a, nb, lds, st);
+        if (ctrl == ABIE_CTRL_PLAIN) return launch_gr15_fp64_plain(cfg, a, nb, lds, st, coop);
+        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_fp64_massive(cfg, a, nb, lds, st, coop);
+        return launch_gr15_fp64_learned(cfg, a, nb, lds, st, coop);
     }
-    if (fm == ABIE_FORCE_FP64_EXACT && ctrl == ABIE_CTRL_PLAIN) return launch_gr15_exact_plain(cfg, a, nb, lds, st);
+    if (fm == ABIE_FORCE_FP64_EXACT && ctrl == ABIE_CTRL_PLAIN) return launch_gr15_exact_plain(cfg, a, nb, lds, st, coop);
     if (fm == ABIE_FORCE_MIXED) {
-        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_mixed_massive(cfg, a, nb, lds, st);
-        if (ctrl == ABIE_CTRL_LEARNED) return launch_gr15_mixed_learned(cfg, a, nb, lds, st);
+        if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_mixed_massive(cfg, a, nb, lds, st, coop);
+        if (ctrl == ABIE_CTRL_LEARNED) return launch_gr15_mixed_learned(cfg, a, nb, lds, st, coop);
     }
     return hipErrorInvalidValue;
 }

@@ -15,9 +15,19 @@ namespace abie {
 // 1 on the unit whose massive bodies are reported.  norm_all = 1 makes a
 // massive-only controller use every body in its norms (systems with at most
 // one massive body, whose massive norm would be 0/0).
+//
+// Linked units (link_n > 1): a system too large for one workgroup under the
+// plain controller runs as link_n slices, units link_base .. link_base +
+// link_n - 1 (link_rank 0 is the primary), each carrying every massive body
+// and a share of the test particles.  The plain controller's norms are over
+// all bodies, so every maximum is combined over the slices through a grid
+// barrier (the launch is cooperative: all slices resident); sys_n is the
+// system's body count and sys_inv0 the offset of its bodies, by ascending
+// original index, in GR15Args::sys_inv.
 struct SysDesc {
     int body0, n, nm, cfg;
     int nt_sys, sys_id, primary, norm_all;
+    int link_n, link_rank, link_base, sys_n, sys_inv0;
 };
 
 // Per-system control block, device-resident; persists across launches.
@@ -31,6 +41,7 @@ struct SysCtl {
     long long n_ce, n_coll;  // event counters (sim.n_close_encounters, ...)
     double prev_h, prev_err, prev_dtreq;  // history used by the step proposer
     int prev_pc, pad;
+    long long sync_epoch;  // grid reductions done by a linked unit (the same on every slice)
 };
 
 enum : int { ST_RUNNING = 0, ST_CE = 1, ST_COLL = 2, ST_DONE = 3, ST_ERROR = -1 };
@@ -69,6 +80,15 @@ struct GR15Args {
     int prefetch;            // units with at least this many massive bodies use the prefetching
                              // source loop (massive-only controllers); 0: never
     int events;        // 0: skip the close-encounter / collision scan
+    // linked units: one arrival counter per system (at its link_base),
+    // partial maxima [2][n_units][2] (two parities), bodies of each linked
+    // system by ascending original index (global body slots), and the
+    // longest a grid barrier may wait before the unit reports ST_ERROR
+    unsigned long long *link_count;
+    double *link_part;
+    const int *sys_inv;
+    int n_units;
+    long long spin_ticks;
 };
 
 // Engine configurations: threads per block and bodies per thread.
@@ -88,7 +108,10 @@ int set_tuning(int n_rules, const int *max_n, const int *cfg);
 size_t gr15_lds_bytes(int cfg);
 
 bool gr15_combo_supported(int force_mode, int ctrl);
-hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st);
+// coop: a cooperative launch (every block resident at once, or an error),
+// for the slices of one linked system.
+hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st,
+                       bool coop = false);
 // flags[0]: hit flag, 0 on entry; flags[1] receives 0, 1 or 2.
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
                          int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, hipStream_t st);

@@ -267,6 +267,68 @@ __device__ __forceinline__ void wg_max2(double &a, double &b, double *red) {
     }
 }
 
+// wg_max2, then (linked units) the maxima over every slice of the system.
+// Thread 0 publishes the slice's pair in the parity-(epoch & 1) half of
+// link_part, releases it, arrives at the system's counter and waits until all
+// link_n slices have arrived for this epoch (the counter only grows:
+// (epoch + 1) * link_n arrivals).  Two parities suffice: a slice can only
+// write the next epoch's pair after every slice arrived at this epoch, i.e.
+// after it read this one.  The wait is bounded by A.spin_ticks of the
+// device's constant-rate clock; false on a timeout (the caller stops the
+// unit with ST_ERROR).  Every slice makes the same sequence of calls.
+// LDS: red[48] carries the wait's outcome, red[49..50] the maxima.
+template <int NT>
+__device__ __forceinline__ bool red_max2(const GR15Args &A, const SysDesc &d, long long &epoch, double &a, double &b, double *red) {
+    wg_max2<NT>(a, b, red);
+    if (d.link_n <= 1) return true;
+    const int tid = threadIdx.x;
+    double *P = A.link_part + ((long long)(epoch & 1) * A.n_units + d.link_base) * 2;
+    unsigned long long *cnt = A.link_count + d.link_base;
+    if (tid == 0) {
+        P[2 * d.link_rank] = a;
+        P[2 * d.link_rank + 1] = b;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long target = (unsigned long long)(epoch + 1) * (unsigned long long)d.link_n;
+        const long long t0 = wall_clock64();
+        double ok = 1.0;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (wall_clock64() - t0 > A.spin_ticks) {
+                ok = 0.0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        red[48] = ok;
+    }
+    __syncthreads();
+    // every wave: the other slices' writes (pairs, event-scan positions) are
+    // visible to the loads that follow
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const bool ok = red[48] != 0.0;
+    if (tid < 64) {
+        double x = -DBL_MAX, y = -DBL_MAX;
+        for (int u = tid; u < d.link_n; u += 64) {
+            x = fmax(x, P[2 * u]);
+            y = fmax(y, P[2 * u + 1]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            x = fmax(x, __shfl_xor(x, off));
+            y = fmax(y, __shfl_xor(y, off));
+        }
+        if (tid == 0) {
+            red[49] = x;
+            red[50] = y;
+        }
+    }
+    __syncthreads();
+    a = red[49];
+    b = red[50];
+    epoch += 1;
+    return ok;
+}
+
 // -------------------------------------------------------------- the kernel
 template <int NT, int BPT>
 struct Bodies {
@@ -470,6 +532,24 @@ __device__ __forceinline__ bool scan_tile(const double *tile, int q0, int cnt, i
     return any;
 }
 
+// x-slab prefilter of scan_tile<false>, two VALU per pair: lanes of the wave
+// with |x_j - x_q| <= cx at some q in [q0, cnt) (q > self on the diagonal
+// tile).  r^2 = fma(dz, dz, fma(dy, dy, dx dx)) >= round(dx dx), so a pair
+// with r^2 <= cd2 has round(dx dx) <= cd2 and |dx| <= cx when cx =
+// sqrt(cd2) (1 + 2^-48) + 2^-500 (the last term covers a dx dx that rounds
+// to zero or to a subnormal): a zero mask means scan_tile<false> over the
+// same tile finds nothing.  cx < 0 on idle lanes, NaN never passes.
+template <bool DIAG>
+__device__ __forceinline__ unsigned long long slab_tile(const double *tile, int q0, int cnt, int self, double x,
+                                                        double cx) {
+    unsigned long long m = 0;
+    for (int q = q0; q < cnt; ++q) {
+        const bool c = fabs(x - tile[4 * q]) <= cx;
+        m |= __ballot(DIAG ? (c && q > self) : c);
+    }
+    return m;
+}
+
 // WPE: waves per SIMD the register allocation targets (512 / WPE registers
 // per lane, AGPRs included): 1 lets a 256-thread group keep 4 bodies per
 // thread in VGPRs + AGPRs; 3 for the one-body-per-thread shapes of 128 to
@@ -514,7 +594,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             b6p[k][c] = A.state[(long long)(45 + c) * T + gi];
         }
     }
-    // events: does any body of the unit have a radius (collisions possible)?
+    // linked slices (plain controller only): maxima over the whole system
+    const bool linked = CT == ABIE_CTRL_PLAIN && d.link_n > 1;
+    const int nall = linked ? d.sys_n : n;  // the system's body count
+    long long epoch = CT == ABIE_CTRL_PLAIN ? ctl->sync_epoch : 0;
+    bool lk_ok = true;
+    auto rmax2 = [&](double &a, double &b) __attribute__((always_inline)) {
+        if (CT == ABIE_CTRL_PLAIN) {
+            if (!red_max2<NT>(A, d, epoch, a, b, E.red)) lk_ok = false;
+        } else {
+            wg_max2<NT>(a, b, E.red);
+        }
+    };
+    // events: does any body of the system have a radius (collisions possible)?
     bool any_rad = false;
     if (A.events) {
         double rmax = 0.0, rdummy = 0.0;
@@ -523,7 +615,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             const int b = k * NT + tid;
             if (b < n) rmax = fmax(rmax, A.rad[b0 + b]);
         }
-        wg_max2<NT>(rmax, rdummy, E.red);
+        rmax2(rmax, rdummy);
         any_rad = rmax > 0.0;
     }
     double t = ctl->t, h = ctl->h;
@@ -543,12 +635,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         for (int k = 0; k < BPT; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (CT == ABIE_CTRL_PLAIN ? (E.slot_valid[k] && 3 * E.slot_orig[k] + c < n)
+                if (CT == ABIE_CTRL_PLAIN ? (E.slot_valid[k] && 3 * E.slot_orig[k] + c < nall)
                                           : (E.norm_all ? (bool)E.slot_valid[k] : (bool)E.slot_massive[k])) {
                     d0 = fmax(d0, fabs(y0[k][c]));
                     d1 = fmax(d1, fabs(a0[k][c]));
                 }
-        wg_max2<NT>(d0, d1, E.red);
+        rmax2(d0, d1);
         const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
 #pragma unroll
         for (int k = 0; k < BPT; ++k)
@@ -562,7 +654,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (E.in_norm(k)) d2 = fmax(d2, fabs(acc[k][c] - a0[k][c]));
-        wg_max2<NT>(d2, dummy, E.red);
+        rmax2(d2, dummy);
         d2 = d2 / dt0;
         double dt1;
         if (fmax(d1, d2) <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
@@ -582,14 +674,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 
     // ---- the step loop (integrator_gauss_radau15.c:279-368)
     double dy[BPT][3];
-    while (n_att < A.max_attempts) {
+    while (lk_ok && n_att < A.max_attempts) {
         // end the launch once its wall-clock budget is spent (the host then
         // relaunches; the state is stored below either way), so that no
         // unit waits on a fixed attempt count of the slowest one
         if (A.budget_ticks > 0 && n_att > 0 && (n_att & 15) == 0) {
             if (tid == 0) flag[2] = (wall_clock64() - t_enter > A.budget_ticks) ? 1.0 : 0.0;
             __syncthreads();
-            if (flag[2] != 0.0) break;
+            double stop = flag[2], unused = 0.0;
+            if (linked) rmax2(stop, unused);  // every slice stops at the same attempt
+            if (stop != 0.0 || !lk_ok) break;
         }
         int pc_used = 0;
         double pc_last = DBL_MAX;
@@ -619,7 +713,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         mdb = fmax(mdb, fabs(bb[k][c][6] - b6p[k][c]));
                         mf = fmax(mf, fabs(acc[k][c]));
                     }
-            wg_max2<NT>(mdb, mf, E.red);
+            rmax2(mdb, mf);
+            if (!lk_ok) break;
             const double pc_err = mdb / mf;
             if (pc_err < kTolpc) break;
             // massive-only controllers: also stop once the correction no
@@ -633,6 +728,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) b6p[k][c] = bb[k][c][6];
         }
+        if (!lk_ok) break;
         n_pc += pc_used;
         n_att += 1;
 
@@ -656,7 +752,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                     mb = fmax(mb, fabs(bb[k][c][6]));
                     ma = fmax(ma, fabs(acc[k][c]));
                 }
-        wg_max2<NT>(mb, ma, E.red);
+        rmax2(mb, ma);
+        if (!lk_ok) break;
         // a system without any acceleration (one body alone, no ext-acc)
         // would give 0/0 as in the reference; it moves freely: estim 0
         const double estim = (ma > 0.0) ? mb / ma : 0.0;
@@ -802,74 +899,130 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         sx[b0 + b] = y0[k][0]; sy[b0 + b] = y0[k][1]; sz[b0 + b] = y0[k][2];
                     }
                 }
+                if (linked) {  // every slice's positions are written before any is read
+                    double u0 = 0.0, u1 = 0.0;
+                    rmax2(u0, u1);
+                    if (!lk_ok) break;
+                }
                 // hit detection: a superset test over each unordered pair
                 // once, bodies staged in LDS tiles of NT (x, y, z, radius);
                 // a hit runs the exact pass below.  The test is symmetric in
-                // the pair, so checking i > j only finds the same hits.
+                // the pair, so checking i > j only finds the same hits.  A
+                // linked slice other than the primary leaves its copies of
+                // the massive bodies out (NaN coordinates never pass), and
+                // also tests its bodies against the slices link_rank + 1 ..
+                // link_rank + link_n / 2 (mod link_n): every pair of slices
+                // is covered once (twice for the opposite slice when link_n
+                // is even, which only repeats the same hits).
                 const double cd = A.ce_dist;
                 const double cd2 = cd * cd * (1.0 + 0x1p-46);
+                const double cx = sqrt(cd2) * (1.0 + 0x1p-48) + 0x1p-500;
+                const bool skipm = linked && !d.primary;
                 double *tile = E.src;  // the force source list is not live here
                 const int lane0 = tid & ~63;
                 bool any = false;
                 double hit = 0.0, dummy2 = 0.0;
+                // the tile [cnt) against the bodies of slot k (q from q0; on
+                // the diagonal tile only entries above the lane's own body)
+                auto test_tile = [&](int k, int q0, int cnt, bool diag) __attribute__((always_inline)) {
+                    const int j = k * NT + tid;
+                    const bool valid = j < n && !(skipm && j < nm);
+                    const double rj = valid ? A.rad[b0 + j] : 0.0;
+                    const double thr0 = valid ? cd2 : -1.0;  // -1: no pair of an idle lane passes
+                    if (any_rad) {
+                        any |= scan_tile<true>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                    } else {
+                        // the full test only for a wave whose x-slab test
+                        // passes somewhere in the tile (wave-uniform branch)
+                        const double cxl = valid ? cx : -1.0;
+                        const unsigned long long m = diag ? slab_tile<true>(tile, q0, cnt, tid, y0[k][0], cxl)
+                                                          : slab_tile<false>(tile, q0, cnt, -1, y0[k][0], cxl);
+                        if (m) any |= scan_tile<false>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                    }
+                };
+                const double qnan = __builtin_nan("");
                 for (int kt = 0; kt < BPT && kt * NT < n; ++kt) {
                     const int base = kt * NT;
                     const int cnt = min(NT, n - base);
                     __syncthreads();
                     if (tid < cnt) {
-                        tile[4 * tid] = y0[kt][0];
-                        tile[4 * tid + 1] = y0[kt][1];
-                        tile[4 * tid + 2] = y0[kt][2];
+                        const bool out = skipm && base + tid < nm;
+                        tile[4 * tid] = out ? qnan : y0[kt][0];
+                        tile[4 * tid + 1] = out ? qnan : y0[kt][1];
+                        tile[4 * tid + 2] = out ? qnan : y0[kt][2];
                         tile[4 * tid + 3] = A.rad[b0 + base + tid];
                     }
                     __syncthreads();
 #pragma unroll
                     for (int k = 0; k < BPT; ++k) {
                         if (k > kt || k * NT + lane0 >= n) continue;  // tile wholly below this wave's bodies
-                        const int j = k * NT + tid;
-                        const bool valid = j < n;
-                        const double rj = valid ? A.rad[b0 + j] : 0.0;
-                        const double thr0 = valid ? cd2 : -1.0;  // -1: no pair of an idle lane passes
                         // on the diagonal tile only i = base + q > j: the wave's
                         // first lane bounds the loop, the rest is a lane mask
                         const bool diag = k == kt;
-                        const int q0 = diag ? lane0 + 1 : 0;
-                        if (any_rad)
-                            any |= scan_tile<true>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
-                        else
-                            any |= scan_tile<false>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                        test_tile(k, diag ? lane0 + 1 : 0, cnt, diag);
+                    }
+                }
+                if (linked) {
+                    for (int off = 1; off <= d.link_n / 2; ++off) {
+                        const int v = (d.link_rank + off) % d.link_n;
+                        const SysDesc dv = A.sys[d.link_base + v];
+                        const long long s0 = dv.body0 + (v == 0 ? 0 : nm);  // the partner without massive copies
+                        const int sc = dv.n - (v == 0 ? 0 : nm);
+                        for (int base = 0; base < sc; base += NT) {
+                            const int cnt = min(NT, sc - base);
+                            __syncthreads();
+                            if (tid < cnt) {
+                                const long long gq = s0 + base + tid;
+                                tile[4 * tid] = sx[gq];
+                                tile[4 * tid + 1] = sy[gq];
+                                tile[4 * tid + 2] = sz[gq];
+                                tile[4 * tid + 3] = A.rad[gq];
+                            }
+                            __syncthreads();
+#pragma unroll
+                            for (int k = 0; k < BPT; ++k) {
+                                if (k * NT + lane0 >= n) continue;
+                                test_tile(k, 0, cnt, false);
+                            }
+                        }
                     }
                 }
                 hit = any ? 1.0 : 0.0;
-                wg_max2<NT>(hit, dummy2, E.red);
+                rmax2(hit, dummy2);
+                if (!lk_ok) break;
                 if (hit > 0.0) {
                     // exact pass in the reference's (j, k) order by the first
-                    // wave: j in original order, the k > j in chunks of 64
-                    // lanes; the ring slots follow from ballots, so the rings
-                    // and counters end as the serial loop leaves them
-                    if (tid < 64) {
+                    // wave (of the primary slice): j in original order, the
+                    // k > j in chunks of 64 lanes; the ring slots follow from
+                    // ballots, so the rings and counters end as the serial
+                    // loop leaves them.  slot(q): the body slot of the q-th
+                    // body of the system by original index.
+                    if (tid < 64 && (!linked || d.primary)) {
+                        auto slot = [&](int q) __attribute__((always_inline)) -> long long {
+                            return linked ? (long long)A.sys_inv[d.sys_inv0 + q] : b0 + A.inv[b0 + q];
+                        };
                         const int lane = tid;
                         const unsigned long long below = (1ull << lane) - 1ull;
                         long long nce = ctl->n_ce, nco = ctl->n_coll;
                         double *rce = (A.ring_ce && A.max_ce > 0) ? A.ring_ce + (long long)sys * 4 * A.max_ce : nullptr;
                         double *rco = (A.ring_coll && A.max_coll > 0) ? A.ring_coll + (long long)sys * 4 * A.max_coll
                                                                        : nullptr;
-                        for (int q1 = 0; q1 < n; ++q1) {
-                            const int je = A.inv[b0 + q1];
-                            const double jo = A.orig[b0 + je];
-                            const double xj = sx[b0 + je], yj = sy[b0 + je], zj = sz[b0 + je];
-                            const double rj = A.rad[b0 + je];
-                            for (int c = q1 + 1; c < n; c += 64) {
+                        for (int q1 = 0; q1 < nall; ++q1) {
+                            const long long sj = slot(q1);
+                            const double jo = A.orig[sj];
+                            const double xj = sx[sj], yj = sy[sj], zj = sz[sj];
+                            const double rj = A.rad[sj];
+                            for (int c = q1 + 1; c < nall; c += 64) {
                                 const int q2 = c + lane;
                                 bool ce = false, co = false;
                                 double dd = 0.0, ko = 0.0;
-                                if (q2 < n) {
-                                    const int ke = A.inv[b0 + q2];
-                                    ko = A.orig[b0 + ke];
-                                    const double dx = xj - sx[b0 + ke], dyy = yj - sy[b0 + ke], dz = zj - sz[b0 + ke];
+                                if (q2 < nall) {
+                                    const long long sk = slot(q2);
+                                    ko = A.orig[sk];
+                                    const double dx = xj - sx[sk], dyy = yj - sy[sk], dz = zj - sz[sk];
                                     const double r2 = dx * dx + dyy * dyy + dz * dz;
                                     dd = sqrt(r2);
-                                    const double rr = rj + A.rad[b0 + ke];
+                                    const double rr = rj + A.rad[sk];
                                     ce = dd <= cd;
                                     co = rr > 0 && dd <= rr;
                                 }
@@ -911,7 +1064,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         }
                     }
                     __syncthreads();
-                    const int st = (int)flag[1];
+                    double stv = (!linked || d.primary) ? flag[1] : -DBL_MAX, unused = 0.0;
+                    if (linked) {  // the primary's outcome reaches every slice
+                        rmax2(stv, unused);
+                        if (!lk_ok) break;
+                    }
+                    const int st = (int)stv;
                     if (st != ST_RUNNING) {
                         status = st;
                         h = h_next;
@@ -922,7 +1080,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         }
         h = h_next;
         if (h < kHmin) {
-            if (!warned && tid == 0)
+            if (!warned && tid == 0 && d.primary)
                 printf("Warning! Timestep %g being too small! Imposing a minimum timestep of %g, t = %g\n", h,
                        kHmin, t);
             warned = 1;
@@ -958,7 +1116,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             ps[(long long)(45 + c) * T + gi] = b6p[k][c];
         }
     }
+    if (!lk_ok) status = ST_ERROR;  // a grid barrier timed out
     if (tid == 0) {
+        if (CT == ABIE_CTRL_PLAIN) ctl->sync_epoch = epoch;
         ctl->t = t;
         ctl->h = h;
         ctl->rejects = rejects;
@@ -977,26 +1137,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 
 
 template <int NT, int BPT, int FM, int CT, int WPE>
-static hipError_t launch_one(const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+static hipError_t launch_one(const GR15Args &a, int nb, size_t lds, hipStream_t st, bool coop) {
+    if (coop) {
+        GR15Args arg = a;
+        void *params[] = {&arg};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&gr15_kernel<NT, BPT, FM, CT, WPE>),
+                                          dim3(nb), dim3(NT), params, (unsigned)lds, st);
+    }
     hipLaunchKernelGGL((gr15_kernel<NT, BPT, FM, CT, WPE>), dim3(nb), dim3(NT), lds, st, a);
     return hipGetLastError();
 }
 
 // Instantiates every configuration of kCfgs for one (force mode, controller).
 template <int FM, int CT>
-hipError_t launch_gr15_fm_ct(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+hipError_t launch_gr15_fm_ct(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st, bool coop) {
     switch (cfg) {
-        case 0: return launch_one<64, 1, FM, CT, 2>(a, nb, lds, st);
-        case 1: return launch_one<128, 1, FM, CT, 3>(a, nb, lds, st);
-        case 2: return launch_one<256, 1, FM, CT, 3>(a, nb, lds, st);
-        case 3: return launch_one<256, 2, FM, CT, 1>(a, nb, lds, st);
-        case 4: return launch_one<512, 1, FM, CT, 3>(a, nb, lds, st);
-        case 5: return launch_one<256, 4, FM, CT, 1>(a, nb, lds, st);
-        case 6: return launch_one<512, 2, FM, CT, 2>(a, nb, lds, st);
-        case 7: return launch_one<1024, 1, FM, CT, 4>(a, nb, lds, st);
-        case 8: return launch_one<512, 3, FM, CT, 2>(a, nb, lds, st);
-        case 9: return launch_one<512, 4, FM, CT, 2>(a, nb, lds, st);
-        case 10: return launch_one<512, 5, FM, CT, 2>(a, nb, lds, st);
+        case 0: return launch_one<64, 1, FM, CT, 2>(a, nb, lds, st, coop);
+        case 1: return launch_one<128, 1, FM, CT, 3>(a, nb, lds, st, coop);
+        case 2: return launch_one<256, 1, FM, CT, 3>(a, nb, lds, st, coop);
+        case 3: return launch_one<256, 2, FM, CT, 1>(a, nb, lds, st, coop);
+        case 4: return launch_one<512, 1, FM, CT, 3>(a, nb, lds, st, coop);
+        case 5: return launch_one<256, 4, FM, CT, 1>(a, nb, lds, st, coop);
+        case 6: return launch_one<512, 2, FM, CT, 2>(a, nb, lds, st, coop);
+        case 7: return launch_one<1024, 1, FM, CT, 4>(a, nb, lds, st, coop);
+        case 8: return launch_one<512, 3, FM, CT, 2>(a, nb, lds, st, coop);
+        case 9: return launch_one<512, 4, FM, CT, 2>(a, nb, lds, st, coop);
+        case 10: return launch_one<512, 5, FM, CT, 2>(a, nb, lds, st, coop);
         default: return hipErrorInvalidValue;
     }
 }

@@ -3,7 +3,7 @@
 #include "gr15_impl.hpp"
 
 namespace abie {
-hipError_t launch_gr15_mixed_massive(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
-    return launch_gr15_fm_ct<ABIE_FORCE_MIXED, ABIE_CTRL_MASSIVE>(cfg, a, nb, lds, st);
+hipError_t launch_gr15_mixed_massive(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st, bool coop) {
+    return launch_gr15_fm_ct<ABIE_FORCE_MIXED, ABIE_CTRL_MASSIVE>(cfg, a, nb, lds, st, coop);
 }
 }  // namespace abie

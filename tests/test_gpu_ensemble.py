@@ -34,11 +34,12 @@ def test_sliced_system_matches_unsliced_subsets():
     assert same_bits(pb[sub], ps[5:]) and same_bits(vb[sub], vs[5:])
 
 
-def test_plain_controller_rejects_oversized_system():
-    """The plain controller needs the whole system in one workgroup (at most
-    2,560 bodies); a larger one is refused with a clear error."""
+def test_plain_controller_rejects_unlinkable_system():
+    """Under the plain controller a system larger than one workgroup runs as
+    linked slices of test particles; 3,000 massive bodies (no test particles
+    to slice) are refused with a clear error."""
     from abie_amd import engine, ics, _lib
-    pos, vel, m = ics.planetary_system(3, 5, 3000)
+    pos, vel, m = ics.planetary_system(3, 3000, 0)
     with pytest.raises(_lib.AbieHipError):
         engine.Ensemble([(pos, vel, m)], controller="plain")
 

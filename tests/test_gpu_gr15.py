@@ -310,6 +310,56 @@ def test_collision_scan_with_radii_across_tiles_matches_oracle(clib, orc, pair):
     assert same_bits(coll, s.buf_coll) and sorted(coll[1:3]) == [i, j]
 
 
+@pytest.mark.parametrize("dup", [True, False])
+@pytest.mark.parametrize("pair", [(4, 900), (600, 601)])
+def test_zero_encounter_distance_scan_matches_oracle(clib, orc, pair, dup):
+    """Close-encounter distance 0 and no radii (the default set-up): a pair
+    of test particles at one position is still a close encounter (d = 0 <=
+    0, common.c:133); a pair that shares its x coordinate at every step
+    (mirror images through the plane of the massive bodies, so x and y stay
+    bitwise equal and z exactly opposite) passes the scan's x-slab
+    prefilter after every step but is no event.  Return code, time, state
+    and ring bit-identical to the oracle's GR15 with exact forces."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(912, 2, 1200)
+    i, j = pair
+    pos = pos.copy()
+    vel = vel.copy()
+    pos[:2, 2] = 0.0
+    vel[:2, 2] = 0.0
+    if dup:
+        pos[j], vel[j] = pos[i], vel[i]
+    else:
+        pos[i, 2] = 0.01
+        mirror = np.array([1.0, 1.0, -1.0])
+        pos[j], vel[j] = pos[i] * mirror, vel[i] * mirror
+    n = len(m)
+    r = np.zeros(n)
+    clib.initialize_code(G, 0.0, n, 1, 1, 0.0, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, 0.05, 1.0)
+    except ParticleException as ex:
+        ret = 1 if "Close" in type(ex).__name__ else 2
+    clib.get_state(p, v, mm, rr)
+    s = orc.Sim(pos, vel, m, r, max_ce=1, max_coll=1, ce_dist=0.0, cr_pow=True)
+    ref = s.integrate(0.05, 0.0)
+    assert ret == ref == (1 if dup else 0)
+    assert clib.get_model_time() == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    ce = clib.get_close_encounter_data().reshape(-1)
+    assert same_bits(ce, s.buf_ce)
+    if dup:
+        assert sorted(ce[1:3]) == [i, j] and ce[3] == 0.0
+    else:
+        q = s.pos.reshape(-1, 3)
+        assert q[i, 0] == q[j, 0] and q[i, 1] == q[j, 1] and q[i, 2] == -q[j, 2] != 0.0
+
+
 def test_many_events_in_one_step_wrap_rings_as_oracle(clib, orc):
     """60 close test-particle pairs, 25 of them colliding, in one step: the
     rings (7 and 3 slots) wrap many times inside one chunk of the parallel

@@ -1,0 +1,149 @@
+"""GR15 under the reference's controller on systems larger than one
+workgroup: the system runs as linked slices that combine every norm over the
+whole system through grid barriers (one cooperative launch).  With exact
+forces each trajectory, step count and event ring must be bit-identical to
+the oracle's GR15 (src/integrator_gauss_radau15.c:221-380, events
+src/common.c:116-158), as for systems that fit one workgroup."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 4 * np.pi ** 2
+
+
+def same_bits(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all((a.view(np.int64) == b.view(np.int64)) | nan))
+
+
+@pytest.fixture()
+def clib():
+    from abie_amd.clibabie import CLibABIE
+    c = CLibABIE()
+    c.lib.finalize_code()
+    yield c
+    c.lib.abie_hip_set_force_mode(0)
+    c.lib.finalize_code()
+
+
+def dropin(clib, pos, vel, m, r, t_end, max_ce=1, max_coll=1, ce_dist=0.0):
+    from abie_amd.events import ParticleException
+    n = len(m)
+    clib.initialize_code(G, 0.0, n, max_ce, max_coll, ce_dist, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, t_end, 1.0)
+    except ParticleException as ex:
+        ret = 1 if "Close" in type(ex).__name__ else 2
+    clib.get_state(p, v, mm, rr)
+    return ret, clib.get_model_time(), p, v
+
+
+@pytest.mark.parametrize("n_test", [2600, 2998])
+def test_linked_dropin_matches_oracle(clib, orc, n_test):
+    """2 massive bodies + 2,600 / 2,998 test particles (the drop-in's slices
+    of at most 128 bodies: 21 / 24 slices; odd and even slice counts pair the
+    slices of the event filter differently): time, state and energy
+    bit-identical to the oracle after 0.1 yr."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(913, 2, n_test)
+    r = np.zeros(len(m))
+    ret, t, p, v = dropin(clib, pos, vel, m, r, 0.1)
+    s = orc.Sim(pos, vel, m, r, max_ce=1, max_coll=1, ce_dist=0.0, cr_pow=True)
+    ref = s.integrate(0.1, 0.0)
+    assert ret == ref == 0
+    assert t == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    assert clib.get_total_energy() == pytest.approx(s.energy(), rel=1e-14)
+
+
+@pytest.mark.parametrize("pair", [(3, 2900), (1000, 1001), (10, 11)])
+def test_linked_close_encounter_across_slices(clib, orc, pair):
+    """A pair of test particles at one position, in different slices, in one
+    slice, and in the primary slice (close-encounter distance 0): return
+    code, time, state and ring as the oracle's."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(914, 3, 2997)
+    pos = pos.copy(); vel = vel.copy()
+    i, j = pair
+    pos[j], vel[j] = pos[i], vel[i]
+    r = np.zeros(len(m))
+    ret, t, p, v = dropin(clib, pos, vel, m, r, 0.1)
+    s = orc.Sim(pos, vel, m, r, max_ce=1, max_coll=1, ce_dist=0.0, cr_pow=True)
+    ref = s.integrate(0.1, 0.0)
+    assert ret == ref == 1
+    assert t == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    ce = clib.get_close_encounter_data().reshape(-1)
+    assert same_bits(ce, s.buf_ce) and sorted(ce[1:3]) == [i, j]
+
+
+def test_linked_collision_with_radii_across_slices(clib, orc):
+    """Radii on every body, a close pair in the first and the 20th slice
+    collides on the first accepted step: both rings and the state as the
+    oracle's."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(915, 2, 2800)
+    pos = pos.copy(); vel = vel.copy()
+    i, j = 5, 2500
+    pos[j] = pos[i] + np.array([3e-4, -2e-4, 1e-4])
+    vel[j] = vel[i]
+    r = np.random.default_rng(5).uniform(1e-7, 1e-6, len(m))
+    r[i] = r[j] = 2e-4
+    ret, t, p, v = dropin(clib, pos, vel, m, r, 0.1, max_ce=2, max_coll=1, ce_dist=1e-3)
+    s = orc.Sim(pos, vel, m, r, max_ce=2, max_coll=1, ce_dist=1e-3, cr_pow=True)
+    ref = s.integrate(0.1, 0.0)
+    assert ret == ref == 2
+    assert t == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
+    coll = clib.get_collision_data().reshape(-1)
+    assert same_bits(coll, s.buf_coll) and sorted(coll[1:3]) == [i, j]
+
+
+def test_linked_relaunches_keep_the_bits(clib, orc, monkeypatch):
+    """A 700-body system with close-in planets (six linked slices: the
+    drop-in links every system of more than 256 bodies); a launch budget of 0.2 ms ends
+    every launch after 16 attempts: the slices stop at the same attempt and
+    the run still matches the oracle."""
+    from abie_amd import ics
+    monkeypatch.setenv("ABIE_HIP_LAUNCH_SECONDS", "0.0002")
+    pos, vel, m = ics.planetary_system(916, 4, 696, planet_a=(0.05, 0.2), tp_a=(0.3, 3.0))
+    r = np.zeros(len(m))
+    ret, t, p, v = dropin(clib, pos, vel, m, r, 0.05)
+    s = orc.Sim(pos, vel, m, r, max_ce=1, max_coll=1, ce_dist=0.0, cr_pow=True)
+    ref = s.integrate(0.05, 0.0)
+    assert s.counters()["attempts"] > 48
+    assert ret == ref == 0
+    assert t == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+
+
+def test_linked_system_in_an_ensemble(orc):
+    """The ensemble API with the plain controller: a 3,000-body system (linked
+    because no workgroup holds it: 12 slices of 256) beside two that fit one
+    workgroup; each bit-identical to the oracle, with the oracle's attempt and
+    step counts."""
+    from abie_amd import engine, ics
+    systems = [ics.planetary_system(917, 3, 20), ics.planetary_system(918, 5, 2995),
+               ics.planetary_system(919, 8, 400)]
+    ens = engine.Ensemble(systems, G=G, force_mode="exact", controller="plain")
+    assert ens.run(0.1)
+    pos, vel, t, st = ens.state()
+    cnt = ens.counters()
+    for k, (p0, v0, m) in enumerate(systems):
+        s = orc.Sim(p0, v0, m, cr_pow=True)
+        s.integrate(0.1, 0.0)
+        p, v = ens.system(k, pos, vel)
+        assert t[k] == s.t
+        assert same_bits(p, s.pos.reshape(-1, 3)), k
+        assert same_bits(v, s.vel.reshape(-1, 3)), k
+        assert cnt[k, 2] == s.counters()["accepted"] and cnt[k, 1] == s.counters()["attempts"]
+    e = ens.energy()
+    assert np.all(np.isfinite(e))
+    ens.close()
