@@ -139,7 +139,10 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     // ensemble's makespan: 4,096 planetary systems of 3..64 massive bodies
     // and 0..2,048 test particles ran 0.2 yr at 512, 811, 1,055 system-yr/s
     // with 1024-, 512- and 256-body slices (128: 1,020).
-    int cap = 256;
+    // A lone system has the GPU to itself: 128-body slices put more CUs on
+    // it (1 star + 4 planets + 100,000 test particles: 133 against 122-125
+    // system-yr/s with 256).
+    int cap = S == 1 ? 128 : 256;
     if (const char *sl = getenv("ABIE_HIP_SLICE")) cap = std::max(128, atoi(sl));
     // plain controller: systems of more than link_above bodies run as linked
     // slices even when one workgroup would hold them (default: only when none

@@ -156,11 +156,13 @@ def cpu_plummer(n=65536):
     return {"interactions_per_s": interactions(n, 0) / dt, "wall_s": dt, "cores": 1, "kind": "reference"}
 
 
-def gr15_dropin(years, force_mode="fp64"):
-    """config-0 system through the reference-compatible C ABI (device=0)."""
+def gr15_dropin(years, force_mode="fp64", n_test=1000):
+    """The 1 star + 4 planets + n_test test particle system (1,000: the
+    configuration-0 system) through the reference-compatible C ABI
+    (device=0), under the reference's own step controller."""
     from abie_amd import ics
     from abie_amd.clibabie import CLibABIE
-    pos, vel, m = ics.config0()
+    pos, vel, m = ics.config0(n_test=n_test)
     c = CLibABIE()
     c.lib.finalize_code()
     c.initialize_code(G, 0.0, len(m), 1, 1, 0.0, deviceID=0)
@@ -348,6 +350,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the 4,096-system ensemble extras (for kernel traces of the GR15 engine)")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="run only the configuration-0 GR15 through the reference ABI, 10 yr "
+                         "(for kernel traces of the linked-slice engine)")
     ap.add_argument("--configs", action="store_true",
                     help="also run the longer per-config measurements (config 0 full 100 yr, "
                          "config 2 1,000-yr mixed run, config 3/4 ensemble samples)")
@@ -370,6 +375,13 @@ def main():
     from abie_amd import _lib, ics
     _lib.load()
     torch.cuda.init()
+
+    if args.dropin_only:
+        rep = gr15_dropin(10.0)
+        log("gr15 drop-in: %s" % rep)
+        json_out.write(json.dumps({"gr15_dropin_config0_10yr": rep}) + "\n")
+        json_out.flush()
+        return
 
     if args.ensemble_only:
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
@@ -454,6 +466,10 @@ def main():
     if args.configs:
         extras["gr15_dropin_config0_100yr"] = gr15_dropin(100.0)
         log("config 0, 100 yr: %s" % extras["gr15_dropin_config0_100yr"])
+        # 100,005 bodies through the reference ABI: linked slices under the
+        # reference's controller (test particles in its error norm)
+        extras["gr15_dropin_config2_plain_2yr"] = gr15_dropin(2.0, n_test=100000)
+        log("config 2 through the reference ABI, 2 yr: %s" % extras["gr15_dropin_config2_plain_2yr"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
         extras["mixed_run_config2"] = mixed_long_run(100000, 1000.0, proposer=w)
         log("config 2 run: %s" % extras["mixed_run_config2"])
