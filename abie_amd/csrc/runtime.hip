@@ -175,9 +175,9 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         // the plain controller's norms cover every body: a system larger than
         // one workgroup runs as linked slices that combine their maxima
         int lcap = link_cap;
-        while (lcap - nm < 64 && lcap < 1024) lcap *= 2;
+        while (lcap - nm < std::min(64, lcap / 2) && lcap < 1024) lcap *= 2;
         const bool link = e->ctrl == ABIE_CTRL_PLAIN && (pick_cfg(n, nm) < 0 || n > link_above) && n > lcap &&
-                          lcap - nm >= 64 && nt > 0 && pick_cfg(lcap, nm) >= 0;
+                          lcap - nm >= std::min(64, lcap / 2) && nt > 0 && pick_cfg(lcap, nm) >= 0;
         if (!slice && !link && pick_cfg(n, nm) >= 0) {
             units.push_back({s, nm, nm, n, 1, 0, -1});
         } else if (slice || link) {
@@ -720,7 +720,7 @@ int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t 
         // 10 yr at 116.6 system-yr/s against 49.7 in one workgroup
         e->link_above = 256;
         e->link_cap = 128;
-        if (const char *sl = getenv("ABIE_HIP_SLICE")) e->link_cap = std::max(128, atoi(sl));
+        if (const char *sl = getenv("ABIE_HIP_SLICE")) e->link_cap = std::max(64, atoi(sl));
         const int offs[2] = {0, (int)N};
         const double *ext = (g.ext_on && g.ext.size() >= 3 * N) ? g.ext.data() : nullptr;
         rc = ens_build(e, offs, pos, vel, m, r, ext);
