@@ -165,6 +165,20 @@ static double orc_pow(const orc_sim *s, double x, double y)
     return s->pow_fn ? s->pow_fn(x, y) : pow(x, y);
 }
 
+/* calculate_accelerations as GR15 calls it.  A deleted body (m < 0) is no
+ * sink: ode_n_body_second_order leaves its entries unwritten
+ * (common.c:48-52), and inside the reference's GR15 those entries are
+ * uninitialised stack arrays (integrator_gauss_radau15.c:234-238), to which
+ * the external acceleration is then added (common.c:107-111).  Here, as on
+ * the device, they are 0 before the external term: a deleted body feels
+ * only the external acceleration. */
+static void gr15_accel(orc_sim *s, const double *pos, const double *vel, double *acc)
+{
+    for (size_t j = 0; j < s->n; j++)
+        if (s->m[j] < 0.0) acc[3 * j] = acc[3 * j + 1] = acc[3 * j + 2] = 0.0;
+    orc_calculate_accelerations(s, pos, vel, acc);
+}
+
 /* integrator_gauss_radau15.c:9-54.  Note the two max-abs calls over only the
  * first n (not 3n) components (:23-24), kept as in the reference. */
 double orc_initial_time_step(orc_sim *s, const double *y0, const double *dy0)
@@ -177,7 +191,7 @@ double orc_initial_time_step(orc_sim *s, const double *y0, const double *dy0)
     double *dy1 = calloc(dim ? dim : 1, sizeof(double));
     double d0, d1, d2, dt0, dt1;
 
-    orc_calculate_accelerations(s, y0, dy0, f0);
+    gr15_accel(s, y0, dy0, f0);
     d0 = orc_vector_max_abs(y0, n);
     d1 = orc_vector_max_abs(f0, n);
     if (d0 < 1e-5 || d1 < 1e-5) dt0 = 1e-6;
@@ -187,7 +201,7 @@ double orc_initial_time_step(orc_sim *s, const double *y0, const double *dy0)
         y1[i] = y0[i] + dt0 * dy0[i];
         dy1[i] = dy0[i] + dt0 * f0[i];
     }
-    orc_calculate_accelerations(s, y1, dy1, f1);
+    gr15_accel(s, y1, dy1, f1);
     d2 = -DBL_MAX;
     for (size_t i = 0; i < dim; i++)
         if (d2 < fabs(f1[i] - f0[i])) d2 = fabs(f1[i] - f0[i]);
@@ -305,7 +319,7 @@ int orc_gr15(orc_sim *s, double t_start, double t_end, double dt_user)
     memcpy(y0, s->pos, dim * sizeof(double));
     memcpy(dy0, s->vel, dim * sizeof(double));
 
-    orc_calculate_accelerations(s, y0, dy0, ddy0);
+    gr15_accel(s, y0, dy0, ddy0);
     h = orc_initial_time_step(s, y0, dy0);
 
     int advance = 1, rejects = 0, warned = 0;
@@ -320,7 +334,7 @@ int orc_gr15(orc_sim *s, double t_start, double t_end, double dt_user)
                 if (ih == 0 && s->skip_first_node)
                     memcpy(ddys, ddy0, dim * sizeof(double));
                 else
-                    orc_calculate_accelerations(s, y, dy, ddys + (size_t)ih * dim);
+                    gr15_accel(s, y, dy, ddys + (size_t)ih * dim);
                 compute_gs(ddys, ih, dim, g);
                 compute_bs_from_gs(g, ih, dim, bs);
             }
@@ -334,7 +348,7 @@ int orc_gr15(orc_sim *s, double t_start, double t_end, double dt_user)
 
         approx_pos(y0, dy0, ddy0, 1., bs, dim, h, y);
         approx_vel(dy0, ddy0, 1., bs, dim, h, dy);
-        orc_calculate_accelerations(s, y, dy, ddy);
+        gr15_accel(s, y, dy, ddy);
 
         const double estim_b6 = orc_vector_max_abs(bs + 6 * dim, dim) / orc_vector_max_abs(ddy, dim);
         const double err = orc_pow(s, estim_b6 / ORC_EPSB, 1. / 7);

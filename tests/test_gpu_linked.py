@@ -147,3 +147,34 @@ def test_linked_system_in_an_ensemble(orc):
     e = ens.energy()
     assert np.all(np.isfinite(e))
     ens.close()
+
+
+def test_linked_external_forces_and_deleted_body(clib, orc):
+    """A 600-body system (five linked slices) with a constant external
+    acceleration on every body (set_additional_forces) and a deleted planet
+    (m < 0: neither source nor sink): state and time bit-identical to the
+    oracle's GR15."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(920, 6, 594)
+    m = m.copy()
+    m[3] = -1.0
+    n = len(m)
+    ext = np.random.default_rng(6).normal(0.0, 1e-6, (n, 3))
+    r = np.zeros(n)
+    clib.initialize_code(G, 0.0, n, 1, 1, 0.0, deviceID=0)
+    clib.lib.abie_hip_set_force_mode(1)
+    p, v, mm, rr = pos.reshape(-1).copy(), vel.reshape(-1).copy(), m.copy(), r.copy()
+    clib.set_state(p, v, mm, rr, n, G, 0.0)
+    clib.set_additional_forces(ext.reshape(-1))
+    ret = 0
+    try:
+        clib.integrator_gr(0.0, 0.3, 1.0)
+    except ParticleException:
+        ret = -1
+    clib.get_state(p, v, mm, rr)
+    s = orc.Sim(pos, vel, m, r, max_ce=1, max_coll=1, ce_dist=0.0, ext_acc=ext, cr_pow=True)
+    ref = s.integrate(0.3, 0.0)
+    assert ret == ref == 0
+    assert clib.get_model_time() == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)

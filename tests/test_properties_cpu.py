@@ -65,3 +65,23 @@ def test_test_particles_do_not_act_and_momentum_is_conserved(orc, sysm):
         assert np.abs(f).max() <= 1e-12 * scale
     # deleted sinks are left untouched (zero here)
     assert np.all(a[m < 0] == 0.0)
+
+
+def test_oracle_gr15_deleted_body_feels_only_the_external_acceleration(orc):
+    """Inside GR15 a deleted body (m < 0) is neither source nor sink; with an
+    external acceleration set it moves under that constant acceleration
+    alone: x(t) = x0 + v0 t + a t^2 / 2 (GR15 integrates a constant
+    acceleration exactly up to rounding).  (Its components still enter the
+    controller's norms, as every body's do in the reference.)"""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(41, 4, 6)
+    m = m.copy()
+    m[2] = -1.0
+    ext = np.zeros((len(m), 3))
+    ext[2] = [3e-3, -2e-3, 1e-3]
+    s = orc.Sim(pos, vel, m, ext_acc=ext)
+    s.integrate(1.0, 0.0)
+    t = s.t
+    want = pos[2] + vel[2] * t + 0.5 * ext[2] * t * t
+    got = s.pos.reshape(-1, 3)[2]
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-13)
