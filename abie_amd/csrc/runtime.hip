@@ -148,6 +148,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     // slices even when one workgroup would hold them (default: only when none
     // does)
     int link_above = e->link_above;
+    if (S == 1 && link_above == (1 << 30)) link_above = 256;  // a lone system: linked slices over more CUs
     if (const char *la = getenv("ABIE_HIP_LINK_ABOVE")) link_above = atoi(la);
     const int link_cap = e->link_cap > 0 ? e->link_cap : cap;
     link_above = std::max(link_above, link_cap);

@@ -178,3 +178,20 @@ def test_linked_external_forces_and_deleted_body(clib, orc):
     assert ret == ref == 0
     assert clib.get_model_time() == s.t
     assert same_bits(p, s.pos) and same_bits(v, s.vel)
+
+
+def test_lone_system_ensemble_links_and_matches_oracle(orc):
+    """The ensemble API with one plain system of 700 bodies links it too (six
+    slices of at most 128): bit-identical to the oracle, with its counts."""
+    from abie_amd import engine, ics
+    pos, vel, m = ics.planetary_system(921, 4, 696)
+    ens = engine.Ensemble([(pos, vel, m)], G=G, force_mode="exact", controller="plain")
+    assert ens.run(0.3)
+    p, v, t, _ = ens.state()
+    cnt = ens.counters()
+    ens.close()
+    s = orc.Sim(pos, vel, m, cr_pow=True)
+    s.integrate(0.3, 0.0)
+    assert t[0] == s.t
+    assert same_bits(p, s.pos.reshape(-1, 3)) and same_bits(v, s.vel.reshape(-1, 3))
+    assert cnt[0, 2] == s.counters()["accepted"] and cnt[0, 1] == s.counters()["attempts"]
