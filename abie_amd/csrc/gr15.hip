@@ -53,6 +53,17 @@ int pick_cfg(int n, int nm) {
 // [0, 32) reductions, [40, 48) broadcast flags, [64, 128) proposer hidden layer.
 size_t gr15_lds_bytes(int cfg) { return (size_t)2 * src_cap(kCfgs[cfg].nt) * 4 * sizeof(double) + 128 * sizeof(double); }
 
+// waves per SIMD each configuration's kernel is built for (the WPE template
+// argument of launch_gr15_fm_ct), 4 SIMDs and 160 KB of LDS per CU
+static const int kCfgWpe[kNumCfg] = {2, 3, 3, 1, 3, 1, 2, 4, 2, 2, 2};
+
+int gr15_blocks_per_cu(int cfg) {
+    if (cfg < 0 || cfg >= kNumCfg) return 0;
+    const int by_waves = 4 * kCfgWpe[cfg] / (kCfgs[cfg].nt / 64);
+    const int by_lds = (int)((size_t)160 * 1024 / gr15_lds_bytes(cfg));
+    return std::min(by_waves, by_lds);
+}
+
 
 // one translation unit per (force mode, controller) pair, compiled in parallel
 hipError_t launch_gr15_fp64_plain(int, const GR15Args &, int, size_t, hipStream_t, bool);

@@ -106,6 +106,9 @@ int max_bodies_one_group();
 // configuration fits it.
 int set_tuning(int n_rules, const int *max_n, const int *cfg);
 size_t gr15_lds_bytes(int cfg);
+// Workgroups of configuration cfg that one CU holds at once (its waves per
+// SIMD target and its LDS): the bound on the slices of one cooperative launch.
+int gr15_blocks_per_cu(int cfg);
 
 bool gr15_combo_supported(int force_mode, int ctrl);
 // coop: a cooperative launch (every block resident at once, or an error),

@@ -151,6 +151,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     if (S == 1 && link_above == (1 << 30)) link_above = 256;  // a lone system: linked slices over more CUs
     if (const char *la = getenv("ABIE_HIP_LINK_ABOVE")) link_above = atoi(la);
     const int link_cap = e->link_cap > 0 ? e->link_cap : cap;
+    const bool auto_link_cap = getenv("ABIE_HIP_SLICE") == nullptr;
+    const int n_cu = device_cu_count(e->dev);
     link_above = std::max(link_above, link_cap);
     // units: a system that fits one workgroup is one unit; with the
     // massive-only controllers a larger system is split into slices that
@@ -175,8 +177,19 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         const bool slice = e->ctrl != ABIE_CTRL_PLAIN && n > cap && nm <= 512 && cap - nm >= 64;
         // the plain controller's norms cover every body: a system larger than
         // one workgroup runs as linked slices that combine their maxima
+        // slice size: keep a linked system to about 256 slices (1 star + 4
+        // planets + 100,000 test particles ran 4.5 / 5.7 / 6.9 / 4.0
+        // system-yr/s with 128 / 256 / 512 / 1,024-body slices; the
+        // configuration-0 system, 1,005 bodies, runs fastest with 128), then
+        // grow it until every slice fits on the GPU at once
         int lcap = link_cap;
+        if (auto_link_cap)
+            while (lcap < 512 && n > 256 * lcap) lcap *= 2;
         while (lcap - nm < std::min(64, lcap / 2) && lcap < 1024) lcap *= 2;
+        auto n_slices = [&](int c) { return (nt + (c - nm) - 1) / std::max(1, c - nm); };
+        while (lcap < 1024 && pick_cfg(lcap, nm) >= 0 &&
+               n_slices(lcap) > n_cu * gr15_blocks_per_cu(pick_cfg(lcap, nm)))
+            lcap *= 2;
         const bool link = e->ctrl == ABIE_CTRL_PLAIN && (pick_cfg(n, nm) < 0 || n > link_above) && n > lcap &&
                           lcap - nm >= std::min(64, lcap / 2) && nt > 0 && pick_cfg(lcap, nm) >= 0;
         if (!slice && !link && pick_cfg(n, nm) >= 0) {
@@ -188,6 +201,9 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
             const int lcfg = link ? pick_cfg(nm + chunk, nm) : -1;  // one kernel for all slices
             if (link && (k < 2 || lcfg < 0))
                 return fail(ABIE_HIP_ETOOBIG, "system %d: %d bodies / %d massive cannot be linked", s, n, nm);
+            if (link && k > n_cu * gr15_blocks_per_cu(lcfg))
+                return fail(ABIE_HIP_ETOOBIG, "system %d: %d bodies need %d linked slices of %d bodies, more "
+                            "than the %d CUs hold at once", s, n, k, nm + chunk, n_cu);
             for (int q = 0; q < k; ++q)
                 units.push_back({s, nm, nm + q * chunk, std::min(n, nm + (q + 1) * chunk), link ? k : 1,
                                  link ? q : 0, lcfg});
