@@ -840,7 +840,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 x[4] = (float)pc_used / 12.0f;
                 x[5] = (float)log10(h_att * sqrt(rate2) + 1e-30);
                 x[6] = (float)log10((double)nm);
-                x[7] = (float)log10((double)d.nt_sys + 1.0);
+                // clamped to the training range (0..2,048 test particles): the
+                // test particles never enter the massive-only error estimate,
+                // so a larger count says nothing more about the step
+                x[7] = (float)fmin(log10((double)d.nt_sys + 1.0), log10(2049.0));
                 x[8] = (float)log10(1.0 + (double)rej_before);
                 x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
                 x[10] = (float)ctl->prev_pc / 12.0f;

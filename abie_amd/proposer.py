@@ -15,7 +15,8 @@ Features (ABIE_MLP_IN = 12, computed identically on the device, see
 gr15_impl.hpp): log10(h_i/dtreq_{i-1}), log10 err_i, log10 err_{i-1},
 log10(h_i/h_{i-1}), pc_i/12, log10(h_i/tau) with tau = min over massive
 pairs of r_ij/|v_ij| (the shortest encounter timescale of the new state),
-log10 n_massive, log10(n_test+1), log10(1 + rejections before step i),
+log10 n_massive, log10(n_test+1) (at most log10 2049, the training range),
+log10(1 + rejections before step i),
 log10(dtreq_i/dtreq_{i-1}), pc_{i-1}/12, log10 of the mass ratio (sum of
 the other massive masses / largest).  Every feature is invariant under a
 change of time unit: an absolute step size is out of distribution near
@@ -50,7 +51,7 @@ def features(rec):
     x[:, 4] = r[:, C_PC] / 12.0
     x[:, 5] = np.log10(h * np.sqrt(r[:, C_RATE2]) + 1e-30)
     x[:, 6] = np.log10(r[:, C_NM])
-    x[:, 7] = np.log10(r[:, C_NT] + 1.0)
+    x[:, 7] = np.minimum(np.log10(r[:, C_NT] + 1.0), np.log10(2049.0))  # the training range, as the engine
     x[:, 8] = np.log10(1.0 + r[:, C_REJ])
     x[:, 9] = np.log10(np.where(pdt > 0, dtreq / np.where(pdt > 0, pdt, 1.0), 1.0))
     x[:, 10] = r[:, C_PPC] / 12.0
