@@ -145,8 +145,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     int cap = S == 1 ? 128 : 256;
     if (const char *sl = getenv("ABIE_HIP_SLICE")) cap = std::max(128, atoi(sl));
     // plain controller: systems of more than link_above bodies run as linked
-    // slices even when one workgroup would hold them (default: only when none
-    // does)
+    // slices even when one workgroup would hold them (default: above 256
+    // bodies for a lone system, otherwise only when no workgroup holds it)
     int link_above = e->link_above;
     if (S == 1 && link_above == (1 << 30)) link_above = 256;  // a lone system: linked slices over more CUs
     if (const char *la = getenv("ABIE_HIP_LINK_ABOVE")) link_above = atoi(la);
@@ -209,8 +209,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
                                  link ? q : 0, lcfg});
         } else {
             return fail(ABIE_HIP_ETOOBIG,
-                        "system %d: %d bodies / %d massive exceeds one workgroup (max %d bodies, %d massive); the "
-                        "massive-only controllers split larger systems",
+                        "system %d: %d bodies / %d massive exceeds one workgroup (max %d bodies, %d massive) and "
+                        "cannot be sliced (a slice carries every massive body plus at least 64 test particles)",
                         s, n, nm, max_bodies_one_group(), src_cap(1024));
         }
     }
