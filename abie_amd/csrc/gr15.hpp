@@ -62,7 +62,7 @@ struct GR15Args {
     const int *inv;                    // the unit's engine indices by ascending original index
     const unsigned char *deleted;      // m < 0
     double *state;                     // [kStateDoubles][total]
-    double *scan;                      // [3][total] event-scan positions
+    double *scan;                      // [4][total] event-scan positions x, y, z and candidate flags
     long long total;
     // events (per system rings, 4 doubles per event)
     double ce_dist;

@@ -536,6 +536,30 @@ __device__ __forceinline__ bool scan_tile(const double *tile, int q0, int cnt, i
     return any;
 }
 
+// scan_tile's test again, for a wave that found a hit in the tile: every
+// pair that passes flags both its bodies (slot sj, and qbase + q for tile
+// entry q) in sf, so that the exact pass visits flagged bodies only (an
+// event pair passes this superset test).  Several lanes may store the same
+// 1.0 to one flag.
+template <bool RAD>
+__device__ __forceinline__ void mark_tile(const double *tile, int q0, int cnt, int self, const double (&y)[3],
+                                          double rj, double thr0, double cd2, double *sf, long long sj,
+                                          long long qbase) {
+    for (int q = q0; q < cnt; ++q) {
+        const double dx = y[0] - tile[4 * q], dy = y[1] - tile[4 * q + 1], dz = y[2] - tile[4 * q + 2];
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        double thr = thr0;
+        if (RAD) {
+            const double rr = rj + tile[4 * q + 3];
+            thr = (rr > 0 && thr0 >= 0) ? fmax(cd2, rr * rr * (1.0 + 0x1p-46)) : thr0;
+        }
+        if ((r2 <= thr) && q > self) {
+            sf[sj] = 1.0;
+            sf[qbase + q] = 1.0;
+        }
+    }
+}
+
 // x-slab prefilter of scan_tile<false>, two VALU per pair: lanes of the wave
 // with |x_j - x_q| <= cx at some q in [q0, cnt) (q > self on the diagonal
 // tile).  r^2 = fma(dz, dz, fma(dy, dy, dx dx)) >= round(dx dx), so a pair
@@ -898,12 +922,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             }
             // close encounters / collisions (common.c:116-158) on the new positions
             if (A.events) {
-                double *sx = A.scan, *sy = A.scan + T, *sz = A.scan + 2 * T;
+                double *sx = A.scan, *sy = A.scan + T, *sz = A.scan + 2 * T, *sf = A.scan + 3 * T;
 #pragma unroll
                 for (int k = 0; k < BPT; ++k) {
                     const int b = k * NT + tid;
                     if (b < n) {
                         sx[b0 + b] = y0[k][0]; sy[b0 + b] = y0[k][1]; sz[b0 + b] = y0[k][2];
+                        sf[b0 + b] = 0.0;
                     }
                 }
                 if (linked) {  // every slice's positions are written before any is read
@@ -931,21 +956,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 double hit = 0.0, dummy2 = 0.0;
                 // the tile [cnt) against the bodies of slot k (q from q0; on
                 // the diagonal tile only entries above the lane's own body)
-                auto test_tile = [&](int k, int q0, int cnt, bool diag) __attribute__((always_inline)) {
+                // qbase: the body slot of tile entry 0
+                auto test_tile = [&](int k, int q0, int cnt, bool diag, long long qbase)
+                                     __attribute__((always_inline)) {
                     const int j = k * NT + tid;
                     const bool valid = j < n && !(skipm && j < nm);
                     const double rj = valid ? A.rad[b0 + j] : 0.0;
                     const double thr0 = valid ? cd2 : -1.0;  // -1: no pair of an idle lane passes
+                    const int self = diag ? tid : -1;
+                    bool hit = false;
                     if (any_rad) {
-                        any |= scan_tile<true>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                        hit = scan_tile<true>(tile, q0, cnt, self, y0[k], rj, thr0, cd2);
+                        if (__ballot(hit)) mark_tile<true>(tile, q0, cnt, self, y0[k], rj, thr0, cd2, sf, b0 + j, qbase);
                     } else {
                         // the full test only for a wave whose x-slab test
                         // passes somewhere in the tile (wave-uniform branch)
                         const double cxl = valid ? cx : -1.0;
                         const unsigned long long m = diag ? slab_tile<true>(tile, q0, cnt, tid, y0[k][0], cxl)
                                                           : slab_tile<false>(tile, q0, cnt, -1, y0[k][0], cxl);
-                        if (m) any |= scan_tile<false>(tile, q0, cnt, diag ? tid : -1, y0[k], rj, thr0, cd2);
+                        if (m) hit = scan_tile<false>(tile, q0, cnt, self, y0[k], rj, thr0, cd2);
+                        if (__ballot(hit)) mark_tile<false>(tile, q0, cnt, self, y0[k], rj, thr0, cd2, sf, b0 + j, qbase);
                     }
+                    any |= hit;
                 };
                 const double qnan = __builtin_nan("");
                 for (int kt = 0; kt < BPT && kt * NT < n; ++kt) {
@@ -966,7 +998,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                         // on the diagonal tile only i = base + q > j: the wave's
                         // first lane bounds the loop, the rest is a lane mask
                         const bool diag = k == kt;
-                        test_tile(k, diag ? lane0 + 1 : 0, cnt, diag);
+                        test_tile(k, diag ? lane0 + 1 : 0, cnt, diag, b0 + base);
                     }
                 }
                 if (linked) {
@@ -989,7 +1021,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
 #pragma unroll
                             for (int k = 0; k < BPT; ++k) {
                                 if (k * NT + lane0 >= n) continue;
-                                test_tile(k, 0, cnt, false);
+                                test_tile(k, 0, cnt, false, s0 + base);
                             }
                         }
                     }
@@ -1016,6 +1048,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                                                                        : nullptr;
                         for (int q1 = 0; q1 < nall; ++q1) {
                             const long long sj = slot(q1);
+                            if (sf[sj] == 0.0) continue;  // no candidate pair (the same for every lane)
                             const double jo = A.orig[sj];
                             const double xj = sx[sj], yj = sy[sj], zj = sz[sj];
                             const double rj = A.rad[sj];
@@ -1023,8 +1056,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                                 const int q2 = c + lane;
                                 bool ce = false, co = false;
                                 double dd = 0.0, ko = 0.0;
-                                if (q2 < nall) {
-                                    const long long sk = slot(q2);
+                                const long long sk = q2 < nall ? slot(q2) : 0;
+                                if (q2 < nall && sf[sk] != 0.0) {
                                     ko = A.orig[sk];
                                     const double dx = xj - sx[sk], dyy = yj - sy[sk], dz = zj - sz[sk];
                                     const double r2 = dx * dx + dyy * dyy + dz * dz;

@@ -317,7 +317,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     HIP_TRY(e->d_inv.reserve(T));
     HIP_TRY(e->d_del.reserve(T));
     HIP_TRY(e->d_state.reserve((size_t)kStateDoubles * T));
-    HIP_TRY(e->d_scan.reserve(3 * T));
+    HIP_TRY(e->d_scan.reserve(4 * T));  // x, y, z of the event scan, then the candidate flags
+    HIP_TRY(hipMemset(e->d_scan.p, 0, 4 * T * sizeof(double)));
     HIP_TRY(e->d_log_len.reserve(1));
     HIP_TRY(e->d_link_count.reserve(U));
     HIP_TRY(e->d_link_part.reserve((size_t)4 * U));
