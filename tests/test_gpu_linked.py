@@ -218,3 +218,30 @@ def test_linked_steplog_and_resume():
     a.close()
     b.close()
     assert ta[0] == tb[0] and same_bits(pa, pb) and same_bits(va, vb)
+
+
+def test_linked_many_events_across_slices_wrap_rings(clib, orc):
+    """60 close test-particle pairs, most of them split across linked slices
+    (partners 600 bodies apart), 25 of them colliding, in one step of a
+    1,400-body system: both rings (7 and 3 slots) wrap many times; rings,
+    counters, time and state as the oracle's."""
+    from abie_amd import ics
+    from abie_amd.events import ParticleException
+    pos, vel, m = ics.planetary_system(923, 2, 1398)
+    pos = pos.copy(); vel = vel.copy()
+    n = len(m)
+    r = np.zeros(n)
+    for q in range(60):
+        a, b = 2 + 11 * q, 2 + 11 * q + 600
+        pos[b] = pos[a] + np.array([2e-4, 1e-4 * (q % 3), -1e-4])
+        vel[b] = vel[a]
+        if q % 12 < 5:
+            r[a] = r[b] = 2e-4
+    ret, t, p, v = dropin(clib, pos, vel, m, r, 0.5, max_ce=7, max_coll=3, ce_dist=1e-3)
+    s = orc.Sim(pos, vel, m, r, max_ce=7, max_coll=3, ce_dist=1e-3, cr_pow=True)
+    ref = s.integrate(0.5, 0.0)
+    assert ret == ref == 1
+    assert t == s.t
+    assert same_bits(p, s.pos) and same_bits(v, s.vel)
+    assert same_bits(clib.get_close_encounter_data().reshape(-1), s.buf_ce)
+    assert same_bits(clib.get_collision_data().reshape(-1), s.buf_coll)
