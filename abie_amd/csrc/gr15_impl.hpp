@@ -369,12 +369,15 @@ struct Engine {
     // the next pair read from LDS while the current one is used (LDS reads
     // return in order, so the wait covers only the older pair).  Pair order
     // and arithmetic are those of the plain loop, so the sums are the same
-    // bits.  At 33-58 massive bodies it shortens a unit's evaluation by ~10 %
-    // when the unit runs alone -- what sets an ensemble's makespan, the few
-    // close-encounter systems with the most steps -- for 3-6 % less
-    // throughput when the CU is shared.  Measured on the C3 ensemble: +10-15 %
-    // under the massive-only controllers, -5 % under the plain one (whole
-    // systems in one workgroup), which keeps the plain loop.
+    // bits.  With two pairs per iteration a lone 256-body unit evaluates
+    // 11-21 % faster (12-58 massive bodies) -- what sets an ensemble's
+    // makespan, the few close-encounter systems with the most steps -- and
+    // 2,048 such units 2-8 % faster (massive-only controllers).  The plain
+    // controller keeps the plain loop: compiled into its kernels this loop
+    // made the configuration-0 drop-in (5 massive bodies, which never use
+    // it) 7 % slower, and the whole-system (512, 2..5) shapes of the plain
+    // ensemble 30 % slower, for 16-21 % on lone one-body-per-thread units
+    // with 8+ massive bodies.
     __device__ __forceinline__ void sum_prefetched(const double *S, const double (&p)[BPT][3],
                                                    double (&acc)[BPT][3]) {
         const int me = wave_massive ? (int)threadIdx.x : -1;
