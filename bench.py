@@ -350,6 +350,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the 4,096-system ensemble extras (for kernel traces of the GR15 engine)")
+    ap.add_argument("--config2-only", action="store_true",
+                    help="run only the 1 star + 4 planets + 100,000 test particle system, 100 yr, "
+                         "FP64 / mixed / learned (for kernel traces)")
     ap.add_argument("--dropin-only", action="store_true",
                     help="run only the configuration-0 GR15 through the reference ABI, 10 yr "
                          "(for kernel traces of the linked-slice engine)")
@@ -375,6 +378,14 @@ def main():
     from abie_amd import _lib, ics
     _lib.load()
     torch.cuda.init()
+
+    if args.config2_only:
+        w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
+        rep = mixed_long_run(100000, 100.0, proposer=w)
+        log("config 2, 100 yr: %s" % rep)
+        json_out.write(json.dumps({"mixed_run_config2_100yr": rep}) + "\n")
+        json_out.flush()
+        return
 
     if args.dropin_only:
         rep = gr15_dropin(10.0)
