@@ -102,17 +102,28 @@ hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hip
 // 256) and streams the bodies k >= 256 b through LDS tiles, testing each
 // unordered pair once with a test that can only over-report (r^2 in the
 // reference's operation order against cd^2 or (r_j + r_k)^2 with a 2^-50
-// slack for the rounding of sqrt); any pass sets *hit.  events_exact_kernel:
-// only if *hit, one wave redoes the reference's ordered (j, k) loop exactly,
-// j serially and the k > j in chunks of 64 lanes, each event's ring slot from
+// slack for the rounding of sqrt); any pass sets *hit, and a wave with a
+// pass in a tile tests that tile again and flags both bodies of every
+// passing pair in evflag (zero on entry).  events_exact_kernel: only if
+// *hit, one wave redoes the reference's ordered (j, k) loop exactly over the
+// flagged bodies (an event pair passes the filter's test, so the skipped
+// pairs hold no event), j serially and the k > j in chunks of 64 lanes, each event's ring slot from
 // a ballot prefix count (an event is skipped when a later one of the same
 // chunk takes its slot), and updates the counters (cnt[0] = close encounters,
 // cnt[1] = collisions) and the rings as the serial loop would.
 constexpr int kEvTile = 256;
 
+__device__ __forceinline__ bool ev_pass(double xj, double yj, double zj, double rj, double tx, double ty,
+                                        double tz, double tr, double cd2) {
+    const double dx = xj - tx, dy = yj - ty, dz = zj - tz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double rr = rj + tr;
+    return (r2 <= cd2) || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50));
+}
+
 __global__ __launch_bounds__(kEvTile) void events_filter_kernel(const double *__restrict__ pos,
                                                                 const double *__restrict__ rad, int n, double cd,
-                                                                int *hit) {
+                                                                int *hit, int *evflag) {
     __shared__ double tx[kEvTile], ty[kEvTile], tz[kEvTile], tr[kEvTile];
     const int j = blockIdx.x * kEvTile + threadIdx.x;
     const bool valid = j < n;
@@ -131,12 +142,17 @@ __global__ __launch_bounds__(kEvTile) void events_filter_kernel(const double *__
             tr[threadIdx.x] = rad[k];
         }
         __syncthreads();
-        for (int q = 0; q < cnt; ++q) {
-            const double dx = xj - tx[q], dy = yj - ty[q], dz = zj - tz[q];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double rr = rj + tr[q];
-            mine |= (k0 + q > j) && ((r2 <= cd2) || (rr > 0 && r2 <= rr * rr * (1.0 + 0x1p-50)));
+        bool tile_hit = false;
+        for (int q = 0; q < cnt; ++q)
+            tile_hit |= valid && (k0 + q > j) && ev_pass(xj, yj, zj, rj, tx[q], ty[q], tz[q], tr[q], cd2);
+        if (__ballot(tile_hit)) {  // rare: flag the passing pairs of this tile
+            for (int q = 0; q < cnt; ++q)
+                if (valid && (k0 + q > j) && ev_pass(xj, yj, zj, rj, tx[q], ty[q], tz[q], tr[q], cd2)) {
+                    evflag[j] = 1;
+                    evflag[k0 + q] = 1;
+                }
         }
+        mine |= tile_hit;
     }
     if (valid && mine) atomicOr(hit, 1);
 }
@@ -144,18 +160,20 @@ __global__ __launch_bounds__(kEvTile) void events_filter_kernel(const double *__
 __global__ __launch_bounds__(64) void events_exact_kernel(const double *__restrict__ pos,
                                                           const double *__restrict__ rad, int n, double t, double cd,
                                                           long long *cnt, int max_ce, int max_coll, double *ring_ce,
-                                                          double *ring_coll, const int *hit, int *ret) {
+                                                          double *ring_coll, const int *hit, int *ret,
+                                                          const int *evflag) {
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
     long long nce = cnt[0], nco = cnt[1];
     if (*hit) {
         for (int j = 0; j < n; ++j) {
+            if (evflag[j] == 0) continue;  // no candidate pair (the same for every lane)
             const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2], rj = rad[j];
             for (int c = j + 1; c < n; c += 64) {
                 const int k = c + lane;
                 bool ce = false, co = false;
                 double d = 0.0;
-                if (k < n) {
+                if (k < n && evflag[k] != 0) {
                     const double dx = xj - pos[3 * k], dy = yj - pos[3 * k + 1], dz = zj - pos[3 * k + 2];
                     d = sqrt(dx * dx + dy * dy + dz * dz);
                     const double rr = rj + rad[k];
@@ -196,16 +214,18 @@ __global__ __launch_bounds__(64) void events_exact_kernel(const double *__restri
     }
 }
 
-// flags[0] must be 0 on entry (hit flag); flags[1] receives 0, 1 or 2.
+// flags[0] must be 0 on entry (hit flag); flags[1] receives 0, 1 or 2;
+// evflag (n ints) must be 0 on entry.
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
-                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, hipStream_t st) {
+                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, int *evflag,
+                         hipStream_t st) {
     if (n <= 0) return hipErrorInvalidValue;
     const int nb = (n + kEvTile - 1) / kEvTile;
-    hipLaunchKernelGGL(events_filter_kernel, dim3(nb), dim3(kEvTile), 0, st, pos, rad, n, cd, flags);
+    hipLaunchKernelGGL(events_filter_kernel, dim3(nb), dim3(kEvTile), 0, st, pos, rad, n, cd, flags, evflag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(events_exact_kernel, dim3(1), dim3(64), 0, st, pos, rad, n, t, cd, cnt, max_ce, max_coll,
-                       ring_ce, ring_coll, flags, flags + 1);
+                       ring_ce, ring_coll, flags, flags + 1, evflag);
     return hipGetLastError();
 }
 

@@ -115,9 +115,11 @@ bool gr15_combo_supported(int force_mode, int ctrl);
 // for the slices of one linked system.
 hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st,
                        bool coop = false);
-// flags[0]: hit flag, 0 on entry; flags[1] receives 0, 1 or 2.
+// flags[0]: hit flag, 0 on entry; flags[1] receives 0, 1 or 2; evflag: n
+// ints of candidate flags, 0 on entry.
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
-                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, hipStream_t st);
+                         int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, int *evflag,
+                         hipStream_t st);
 hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,
                          const double *vx, const double *vy, const double *vz, const double *m, double G,
                          double *energy, hipStream_t st);

@@ -981,12 +981,12 @@ size_t check_collisions_close_encounters(const double *pos, const double *radii,
     if (N == 0) return 0;
     DBuf<double> dpos, drad, dce, dco;
     DBuf<long long> dcnt;
-    DBuf<int> dret;
+    DBuf<int> dret, dflag;
     long long cnt[2] = {(long long)g.n_ce, (long long)g.n_coll};
     int ret = 0;
     bool ok = dpos.reserve(3 * N) == hipSuccess && drad.reserve(N) == hipSuccess && dcnt.reserve(2) == hipSuccess &&
               dret.reserve(2) == hipSuccess && dce.reserve(4 * std::max<size_t>(g.max_ce, 1)) == hipSuccess &&
-              dco.reserve(4 * std::max<size_t>(g.max_coll, 1)) == hipSuccess;
+              dco.reserve(4 * std::max<size_t>(g.max_coll, 1)) == hipSuccess && dflag.reserve(N) == hipSuccess;
     const int flags0[2] = {0, 0};
     ok = ok && hipMemcpy(dret.p, flags0, sizeof flags0, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(dpos.p, pos, 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
@@ -994,14 +994,15 @@ size_t check_collisions_close_encounters(const double *pos, const double *radii,
          hipMemcpy(dcnt.p, cnt, sizeof cnt, hipMemcpyHostToDevice) == hipSuccess &&
          (g.max_ce == 0 || hipMemcpy(dce.p, g.buf_ce.data(), 4 * g.max_ce * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) &&
          (g.max_coll == 0 || hipMemcpy(dco.p, g.buf_coll.data(), 4 * g.max_coll * sizeof(double), hipMemcpyHostToDevice) == hipSuccess);
-    ok = ok && launch_events(dpos.p, drad.p, (int)N, t, g.ce_dist, dcnt.p, (int)g.max_ce, (int)g.max_coll, dce.p, dco.p,
-                             dret.p, g.stream) == hipSuccess &&
+    ok = ok && hipMemsetAsync(dflag.p, 0, N * sizeof(int), g.stream) == hipSuccess &&
+         launch_events(dpos.p, drad.p, (int)N, t, g.ce_dist, dcnt.p, (int)g.max_ce, (int)g.max_coll, dce.p, dco.p,
+                       dret.p, dflag.p, g.stream) == hipSuccess &&
          hipStreamSynchronize(g.stream) == hipSuccess &&
          hipMemcpy(cnt, dcnt.p, sizeof cnt, hipMemcpyDeviceToHost) == hipSuccess &&
          hipMemcpy(&ret, dret.p + 1, sizeof ret, hipMemcpyDeviceToHost) == hipSuccess &&
          (g.max_ce == 0 || hipMemcpy(g.buf_ce.data(), dce.p, 4 * g.max_ce * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) &&
          (g.max_coll == 0 || hipMemcpy(g.buf_coll.data(), dco.p, 4 * g.max_coll * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess);
-    dpos.release(); drad.release(); dce.release(); dco.release(); dcnt.release(); dret.release();
+    dpos.release(); drad.release(); dce.release(); dco.release(); dcnt.release(); dret.release(); dflag.release();
     if (!ok) return (size_t)(long long)fail(ABIE_HIP_EHIP, "check_collisions_close_encounters: device call failed");
     g.n_ce = (size_t)cnt[0];
     g.n_coll = (size_t)cnt[1];
