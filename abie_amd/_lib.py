@@ -54,13 +54,14 @@ SIGNATURES = {
     "abie_hip_set_force_mode": (c_i, [c_i]),
     "abie_hip_accel_soa": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp, c_i, c_vp]),
     "abie_hip_time_accel_soa": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp,
-                                      c_i, c_vp, c_i, c_dp, c_dp]),
+                                      c_i, c_vp, c_i, c_dp, c_dp, c_dp]),
     "abie_ensemble_create": (c_i, [ctypes.POINTER(c_vp), c_i, c_ip, c_dp, c_dp, c_dp, c_dp, c_d, c_i, c_i]),
     "abie_ensemble_destroy": (None, [c_vp]),
     "abie_ensemble_run": (c_i, [c_vp, c_d, c_d, c_ip]),
     "abie_ensemble_get_state": (c_i, [c_vp, c_dp, c_dp, c_dp, c_ip]),
     "abie_ensemble_get_counters": (c_i, [c_vp, c_llp]),
     "abie_ensemble_energy": (c_i, [c_vp, c_dp]),
+    "abie_ensemble_get_finish_times": (c_i, [c_vp, c_dp]),
     "abie_ensemble_set_steplog": (c_i, [c_vp, ctypes.c_longlong]),
     "abie_ensemble_get_steplog": (c_i, [c_vp, c_dp, c_llp]),
     "abie_ensemble_set_proposer": (c_i, [c_vp, c_fp, c_i]),
@@ -76,7 +77,7 @@ SIGNATURES = {
 # constants of include/abie_hip.h
 FORCE_FP64, FORCE_FP64_EXACT, FORCE_MIXED, FORCE_FP32 = 0, 1, 2, 3
 CTRL_PLAIN, CTRL_MASSIVE, CTRL_LEARNED = 0, 1, 2
-STEPLOG_FIELDS = 18
+STEPLOG_FIELDS = 31
 MLP_IN, MLP_HIDDEN = 12, 64
 MLP_NPARAM = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
 

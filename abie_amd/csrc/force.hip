@@ -450,7 +450,14 @@ __global__ void unpack_acc(const int *__restrict__ perm, const double *__restric
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int i = perm[p];
-    if (m[i] < 0.0) return;  // deleted body: the caller's value stays (common.c:49-52)
+    if (m[i] < 0.0) {
+        // deleted body: gravity leaves the caller's value (common.c:49-52),
+        // the external acceleration is still added to it (common.c:107-111)
+        if (ext) {
+            acc[3 * i] += ext[3 * i]; acc[3 * i + 1] += ext[3 * i + 1]; acc[3 * i + 2] += ext[3 * i + 2];
+        }
+        return;
+    }
     double vx = ax[p], vy = ay[p], vz = az[p];
     if (ext) {
         vx += ext[3 * i]; vy += ext[3 * i + 1]; vz += ext[3 * i + 2];

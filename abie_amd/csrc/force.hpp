@@ -22,6 +22,10 @@ struct AccelPlan {
 };
 
 AccelPlan plan_accel(int n_src, int n_sink, int mode, int n_cu);
+// Doubles of source-split scratch a plan needs (0 when it does not split).
+inline size_t plan_scratch_doubles(const AccelPlan &pl, int n_sink) {
+    return pl.parts > 1 ? (size_t)pl.parts * 3 * (size_t)n_sink : 0;
+}
 // `part` must hold parts*3*n_sink doubles when plan.parts > 1.
 hipError_t launch_accel(const AccelArgs &a, const AccelPlan &plan, double *part, hipStream_t st);
 

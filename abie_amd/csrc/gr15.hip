@@ -50,7 +50,8 @@ int pick_cfg(int n, int nm) {
 }
 
 // LDS: sources [2][NT] x {x, y, z, gm}, then 128 doubles of scratch:
-// [0, 32) reductions, [40, 48) broadcast flags, [64, 128) proposer hidden layer.
+// [0, 32) reductions, [40, 51) broadcast flags and linked-slice maxima,
+// [64, 96) proposer hidden layer, [96, 109) proposer inputs and output.
 size_t gr15_lds_bytes(int cfg) { return (size_t)2 * src_cap(kCfgs[cfg].nt) * 4 * sizeof(double) + 128 * sizeof(double); }
 
 // waves per SIMD each configuration's kernel is built for (the WPE template

@@ -42,6 +42,10 @@ struct SysCtl {
     double prev_h, prev_err, prev_dtreq;  // history used by the step proposer
     int prev_pc, pad;
     long long sync_epoch;  // grid reductions done by a linked unit (the same on every slice)
+    // device wall-clock ticks (constant-rate clock) of the unit's first
+    // launch of the current run and of the moment it stopped running; 0
+    // until set (cleared by the host at the start of each run)
+    long long start_tick, done_tick;
 };
 
 enum : int { ST_RUNNING = 0, ST_CE = 1, ST_COLL = 2, ST_DONE = 3, ST_ERROR = -1 };

@@ -485,11 +485,11 @@ struct Engine {
     }
 };
 
-// Features of the learned step proposer (ABIE_MLP_IN values), computed by
-// every thread from LDS-broadcast system scalars.
-struct PropFeat {
-    float f[ABIE_MLP_IN];
-};
+// LDS scratch (doubles after the source lists, gr15_lds_bytes): [0, 32)
+// reductions, [40, 48) flags broadcast by thread 0, [48, 51) linked-slice
+// maxima, [64, 96) the proposer's hidden layer (64 floats), [96, 109) the
+// proposer's inputs and output kept for the step log.
+constexpr int kLdsFeat = 96;
 
 __device__ __forceinline__ float mlp_eval(const float *__restrict__ w, const float *x, float *lds_h) {
     // 12 -> 64 (ReLU) -> 64 (ReLU) -> 1; one wave, lane = hidden unit
@@ -880,7 +880,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 x[10] = (float)ctl->prev_pc / 12.0f;
                 x[11] = (float)log10(A.mratio[sys] + 1e-12);
                 const float yl = mlp_eval(A.mlp, x, (float *)(E.red + 64));
-                if (tid == 0) flag[0] = (double)yl;
+                if (tid == 0) {
+                    flag[0] = (double)yl;
+                    if (A.log != nullptr) {  // the network's inputs and raw output, for the step log
+#pragma unroll
+                        for (int i = 0; i < ABIE_MLP_IN; ++i) E.red[kLdsFeat + i] = (double)x[i];
+                        E.red[kLdsFeat + ABIE_MLP_IN] = (double)yl;
+                    }
+                }
             }
             __syncthreads();
             double ratio = exp10(fmin(fmax(flag[0], kPropLo), kPropHi));
@@ -906,6 +913,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
                 r[6] = accepted; r[7] = pc_used; r[8] = rej_before; r[9] = ctl->prev_h; r[10] = ctl->prev_err;
                 r[11] = ctl->prev_dtreq; r[12] = ctl->prev_pc; r[13] = nm; r[14] = d.nt_sys; r[15] = h_next;
                 r[16] = rate2; r[17] = A.mratio[sys];
+                // proposer inputs and raw output (NaN when no proposal was made)
+                const bool proposed = CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr;
+#pragma unroll
+                for (int i = 0; i <= ABIE_MLP_IN; ++i) r[18 + i] = proposed ? E.red[kLdsFeat + i] : __builtin_nan("");
             }
         }
         if (accepted) {
@@ -1175,6 +1186,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         ctl->n_pc += n_pc;
         ctl->n_prop_used += n_pu;
         ctl->n_prop_rej += n_pr;
+        if (ctl->start_tick == 0) ctl->start_tick = t_enter;
+        if (status != ST_RUNNING) ctl->done_tick = wall_clock64();
     }
 }
 

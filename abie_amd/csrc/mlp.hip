@@ -172,21 +172,6 @@ __global__ __launch_bounds__(256) void adam_kernel(float *w, const float *g, flo
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-template <typename T>
-struct WBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    bool reserve(size_t want) {
-        if (want <= n) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc((void **)&p, want * sizeof(T)) != hipSuccess) return false;
-        n = want;
-        return true;
-    }
-};
-
 }  // namespace abie
 
 using namespace abie;
@@ -203,13 +188,15 @@ int abie_mlp_forward(const float *w, const float *x, float *y, int n, void *stre
 int abie_mlp_loss_grad(const float *w, const float *x, const float *target, const float *sw, int n, float *grad,
                        float *loss, void *stream) {
     if (!w || !x || !target || !grad || !loss || n <= 0) return ABIE_HIP_EINVAL;
-    static thread_local WBuf<float> ws;
     const int chunks = (n + CH - 1) / CH;
     const size_t need = (size_t)4 * n * H + 2 * (size_t)n + (size_t)chunks * NPARAM + 1;
-    if (!ws.reserve(need)) return ABIE_HIP_ENOMEM;
-    float *H1 = ws.p, *H2 = H1 + (size_t)n * H, *dH1 = H2 + (size_t)n * H, *dH2 = dH1 + (size_t)n * H;
-    float *dY = dH2 + (size_t)n * H, *L = dY + n, *part = L + n, *inv = part + (size_t)chunks * NPARAM;
     hipStream_t st = (hipStream_t)stream;
+    // stream-ordered workspace: concurrent calls on different streams each
+    // get their own, and the pool reuses it once the stream passes the free
+    float *ws = nullptr;
+    if (hipMallocAsync((void **)&ws, need * sizeof(float), st) != hipSuccess) return ABIE_HIP_ENOMEM;
+    float *H1 = ws, *H2 = H1 + (size_t)n * H, *dH1 = H2 + (size_t)n * H, *dH2 = dH1 + (size_t)n * H;
+    float *dY = dH2 + (size_t)n * H, *L = dY + n, *part = L + n, *inv = part + (size_t)chunks * NPARAM;
     hipLaunchKernelGGL(mlp_wsum_kernel, dim3(1), dim3(64), 0, st, sw, n, inv);
     hipLaunchKernelGGL(mlp_sample_kernel, dim3((n + TB - 1) / TB), dim3(TB), 0, st, w, x, target, sw, inv, n, H1, H2,
                        dH1, dH2, dY, L);
@@ -217,7 +204,9 @@ int abie_mlp_loss_grad(const float *w, const float *x, const float *target, cons
                        dY, n, part);
     hipLaunchKernelGGL(mlp_sum_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, part, chunks, grad);
     hipLaunchKernelGGL(mlp_loss_kernel, dim3(1), dim3(256), 0, st, L, n, loss);
-    return hipGetLastError() == hipSuccess ? 0 : ABIE_HIP_EHIP;
+    const hipError_t le = hipGetLastError();
+    const hipError_t fe = hipFreeAsync(ws, st);
+    return (le == hipSuccess && fe == hipSuccess) ? 0 : ABIE_HIP_EHIP;
 }
 
 int abie_mlp_adam(float *w, const float *g, float *m, float *v, int n_param, int step, float lr, float beta1,

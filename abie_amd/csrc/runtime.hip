@@ -440,6 +440,7 @@ static int ens_set_t_end(abie_ensemble *e, double t_end, const double *t_start) 
         ctl[u].t_end = t_end;
         if (t_start) ctl[u].t = t_start[e->unit_sys[u]];
         if (ctl[u].status == ST_DONE && ctl[u].t <= t_end) ctl[u].status = ST_RUNNING;
+        ctl[u].start_tick = ctl[u].done_tick = 0;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), e->n_units * sizeof(SysCtl), hipMemcpyHostToDevice));
     return 0;
@@ -595,6 +596,26 @@ int abie_ensemble_get_counters(abie_ensemble *e, long long *cnt) {
     return 0;
 }
 
+int abie_ensemble_get_finish_times(abie_ensemble *e, double *seconds) {
+    if (!e || !seconds) return fail(ABIE_HIP_EINVAL, "ensemble_get_finish_times: bad arguments");
+    std::vector<SysCtl> ctl(e->n_units);
+    HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+    long long t0 = 0;
+    for (auto &c : ctl)
+        if (c.start_tick > 0 && (t0 == 0 || c.start_tick < t0)) t0 = c.start_tick;
+    const double khz = e->clock_khz > 0 ? (double)e->clock_khz : 1.0;
+    for (int s = 0; s < e->n_sys; ++s) seconds[s] = 0.0;
+    for (int u = 0; u < e->n_units; ++u) {
+        const SysCtl &c = ctl[u];
+        // a unit that stopped before this run (its launches return at once) counts as 0
+        const double f = (c.done_tick > 0 && t0 > 0) ? (double)(c.done_tick - t0) / (khz * 1e3)
+                         : (c.status == ST_RUNNING ? -1.0 : 0.0);
+        double &d = seconds[e->unit_sys[u]];
+        d = (f < 0 || d < 0) ? -1.0 : std::max(d, f);
+    }
+    return 0;
+}
+
 int abie_ensemble_energy(abie_ensemble *e, double *energy) {
     if (!e || !energy) return fail(ABIE_HIP_EINVAL, "ensemble_energy: bad arguments");
     HIP_TRY(launch_energy(e->d_sys.p, e->n_units, e->d_x.p, e->d_y.p, e->d_z.p, e->d_vx.p, e->d_vy.p, e->d_vz.p,
@@ -727,6 +748,14 @@ int run_gr15(double *pos, double *vel, const double *m, const double *r, size_t 
              double t_end, double *t_out, bool cache) {
     if (N == 0) return fail(ABIE_HIP_EINVAL, "integrator_gr: no bodies (the reference never terminates here)");
     if (g.C > 0.0) return fail(ABIE_HIP_EUNSUP, "post-Newtonian forces (C > 0) are not part of the device path");
+    // The reference's controller takes its norms over every body, test
+    // particles included, and the seventh divided difference of its error
+    // estimate amplifies single-precision noise in their forces by ~1e4: the
+    // mixed mode is built for the massive-only controllers of the ensemble
+    // API only.  calculate_accelerations still honours it.
+    if (g.force_mode == ABIE_FORCE_MIXED)
+        return fail(ABIE_HIP_EUNSUP, "integrator_gr: mixed forces run only under the massive-only controllers "
+                    "(abie_ensemble_*); the reference's controller takes fp64 or exact forces");
     int rc = check_device();
     if (rc) return rc;
     abie_ensemble *e = cache ? g.gr_cache : nullptr;
@@ -1057,18 +1086,30 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz, con
     if (mode < 0 || mode > 3) return fail(ABIE_HIP_EINVAL, "accel_soa: bad mode %d", mode);
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
-    static thread_local DBuf<double> part;
     AccelArgs a{sx, sy, sz, gm, px, py, pz, ax, ay, az, n_src, n_sink, n_self, mode};
     AccelPlan pl = plan_accel(n_src, n_sink, mode, device_cu_count(dev));
-    if (pl.parts > 1) HIP_TRY(part.reserve((size_t)pl.parts * 3 * n_sink));
-    HIP_TRY(launch_accel(a, pl, part.p, (hipStream_t)stream));
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t need = plan_scratch_doubles(pl, n_sink);
+    if (need == 0) {
+        HIP_TRY(launch_accel(a, pl, nullptr, st));
+        return 0;
+    }
+    // The source-split partials live in a stream-ordered allocation: calls on
+    // different streams get their own buffers, and the pool reuses the
+    // memory once the stream has passed the free.
+    double *part = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&part, need * sizeof(double), st));
+    const hipError_t le = launch_accel(a, pl, part, st);
+    const hipError_t fe = hipFreeAsync(part, st);
+    if (le != hipSuccess) return fail(ABIE_HIP_EHIP, "accel_soa: launch failed: %s", hipGetErrorString(le));
+    HIP_TRY(fe);
     return 0;
 }
 
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz, const double *gm, int n_src,
                             const double *px, const double *py, const double *pz, int n_sink, int n_self,
                             double *ax, double *ay, double *az, int mode, void *stream, int reps,
-                            double *ms_per_launch, double *ms_each) {
+                            double *ms_per_launch, double *ms_each, double *ms_each_median) {
     if (reps <= 0 || !ms_per_launch) return fail(ABIE_HIP_EINVAL, "time_accel_soa: bad arguments");
     hipStream_t st = (hipStream_t)stream;
     std::vector<hipEvent_t> ev(ms_each ? 2 * reps : 2);
@@ -1100,13 +1141,19 @@ int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz
             return rc;
         }
         HIP_TRY(hipEventSynchronize(ev[2 * reps - 1]));
+        std::vector<double> each(reps);
         double tot = 0.0;
         for (int r = 0; r < reps; ++r) {
             float m1 = 0;
             HIP_TRY(hipEventElapsedTime(&m1, ev[2 * r], ev[2 * r + 1]));
+            each[r] = m1;
             tot += m1;
         }
         *ms_each = tot / reps;
+        if (ms_each_median) {
+            std::sort(each.begin(), each.end());
+            *ms_each_median = reps % 2 ? each[reps / 2] : 0.5 * (each[reps / 2 - 1] + each[reps / 2]);
+        }
     }
     destroy();
     return 0;

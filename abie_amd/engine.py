@@ -62,19 +62,22 @@ def accel_soa(src, sinks, n_self, mode="fp64", out=None, stream=None):
 def time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=100, stream=None, each=False):
     """Mean milliseconds per launch of ``reps`` back-to-back launches, timed
     with HIP events recorded on the launch stream.  each=True also returns
-    the mean duration of a launch bracketed by its own event pair."""
+    the mean and the median duration of a launch bracketed by its own event
+    pair: (back_to_back, each_mean, each_median)."""
     sx, sy, sz, gm = src
     px, py, pz = sinks
     ax, ay, az = out
     ms = ctypes.c_double(0.0)
     ms1 = ctypes.c_double(0.0)
+    med = ctypes.c_double(0.0)
     lib = _lib.load()
     _lib.check(lib.abie_hip_time_accel_soa(_ptr(sx), _ptr(sy), _ptr(sz), _ptr(gm), sx.numel(), _ptr(px), _ptr(py),
                                            _ptr(pz), px.numel(), int(n_self), _ptr(ax), _ptr(ay), _ptr(az),
                                            _mode(mode), _stream(stream), int(reps), ctypes.byref(ms),
-                                           ctypes.byref(ms1) if each else None),
+                                           ctypes.byref(ms1) if each else None,
+                                           ctypes.byref(med) if each else None),
                "abie_hip_time_accel_soa")
-    return (ms.value, ms1.value) if each else ms.value
+    return (ms.value, ms1.value, med.value) if each else ms.value
 
 
 def soa_from_bodies(pos, m, G, device="cuda"):
@@ -154,6 +157,13 @@ class Ensemble:
         _lib.check(self.lib.abie_ensemble_get_counters(self.h, c.ctypes.data_as(_lib.c_llp)),
                    "abie_ensemble_get_counters")
         return c
+
+    def finish_times(self):
+        """Seconds from the first unit's start to each system's end in the
+        last run (device clock); -1 for a system still running."""
+        f = np.zeros(self.n_sys)
+        _lib.check(self.lib.abie_ensemble_get_finish_times(self.h, _dp(f)), "abie_ensemble_get_finish_times")
+        return f
 
     def energy(self):
         e = np.zeros(self.n_sys)

@@ -36,6 +36,9 @@ EPS = 1e-300
 # step-log columns (gr15_impl.hpp)
 (C_SYS, C_T, C_H, C_EST, C_ERR, C_DTREQ, C_ACC, C_PC, C_REJ, C_PH, C_PERR, C_PDT, C_PPC, C_NM, C_NT, C_HNEXT,
  C_RATE2, C_MRATIO) = range(18)
+# the proposer's inputs as the kernel evaluated them ([C_FEAT, C_FEAT + 12))
+# and its raw output; NaN on attempts without a proposal
+C_FEAT, C_PROP_Y = 18, 30
 
 
 def features(rec):

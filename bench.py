@@ -6,23 +6,28 @@ A "step" is one acceleration evaluation of the force-kernel workload: 64
 massive bodies (star + 63, m log-U[1e-9, 1e-3]) plus 1,048,576 test
 particles (a ~ U[1, 100] au), FP64, inputs resident in HBM.  value =
 algorithmic interactions (N_m (N_m - 1) + N_t N_m = 67,112,896 per step) per
-second over the K timed steps.  The kernel's mean launch time comes from HIP
-events on the launch stream (roofline.achieved), its HBM traffic from the
-committed rocprofv3 counter summary (profiles/force_c1a_traffic.json).
+second over the K timed steps.  The kernel's mean launch time
+(roofline.achieved) comes from HIP events on the launch stream around at
+least 20 ms of back-to-back launches, whatever K is; its HBM traffic from
+the committed rocprofv3 counter summary (profiles/force_c1a_traffic.json).
 
 Extras (reported beside the headline, not part of it): the 65,536-body
-Plummer force case, the mixed-precision and exact force modes, the drop-in
-GR15 on the 1 star + 4 planets + 1,000 test particle system (GPU, and the
+Plummer force case (and the fast FP64 mode's difference from the exact mode
+on every sink), the mixed-precision and exact force modes, the drop-in GR15
+on the 1 star + 4 planets + 1,000 test particle system (GPU, and the
 reference's own C GR15 on one host core), the mixed-precision force error on
-that system with 100,000 test particles, and the 4,096-system ensemble
-(0.2 yr per system) under the plain FP64 GR15, the massive-only controller
-and the learned step proposer (FP64 and mixed test forces).  --configs adds
-the 100-yr drop-in run, the 1,000-yr mixed run with 100,000 test particles
-and a 1-yr ensemble pass.
+that system with 100,000 test particles, the 4,096-system ensemble (0.2 yr
+per system) under the plain FP64 GR15, the massive-only controller and the
+learned step proposer (FP64 and mixed test forces), and a 10-yr pass of the
+plain GR15 (stopped at 90 s) and of the chosen form.  --configs adds the
+100-yr drop-in run, the 1,000-yr mixed run with 100,000 test particles and a
+1-yr ensemble pass.
 
 The CPU baseline times the reference's own C force loop (oracle/_ref, built
 from the reference sources) -- or the oracle's restatement when that is
-absent -- on one host core, on a bounded sample (64 + 65,536 test particles).
+absent -- on one host core, on a bounded sample (64 + 65,536 test
+particles), and extrapolates it to the 64 + 1,048,576 workload by the N^2
+loop count of the reference's loop.
 """
 import argparse
 import json
@@ -38,6 +43,9 @@ sys.path.insert(0, ROOT)
 G = 4 * np.pi ** 2
 FLOP_PER_INTERACTION = 20          # 3 sub, 6 for r^2, 4 for r^-3, 1 for m r^-3, 6 accumulate
 FP64_PEAK_TFLOPS = 78.6            # MI355X FP64 vector peak
+# the FP64 interaction issues 16 VALU ops + 1 v_rsq_f64 at quarter rate: 20
+# issue slots against the 10 FMA slots the peak assumes for 20 FLOP
+ISSUE_CEILING_FRAC = 10.0 / 20.0
 METRIC = "interactions/s & system-yr/s at |dE/E|<=1e-10, chosen form vs plain FP64 GR15"
 
 
@@ -49,10 +57,13 @@ def interactions(n_m, n_t):
     return n_m * (n_m - 1) + n_t * n_m
 
 
-def force_case(pos, m, Gc, mode, steps, warmup, each=False):
+def force_case(pos, m, Gc, mode, steps, warmup, each=False, roofline_ms=20.0):
     """Warm up, then time ``steps`` back-to-back launches (wall clock between
-    device synchronisations, and HIP events on the launch stream); each=True
-    adds a pass with every launch bracketed by its own event pair."""
+    device synchronisations, and HIP events on the launch stream).  each=True
+    adds the roofline timing, independent of ``steps``: back-to-back launches
+    for at least ``roofline_ms`` of device time (HIP events on the launch
+    stream around the whole run), and 200 launches each bracketed by its own
+    event pair (mean and median)."""
     import torch
     from abie_amd import engine
     src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, Gc)
@@ -66,8 +77,11 @@ def force_case(pos, m, Gc, mode, steps, warmup, each=False):
     wall = time.perf_counter() - t0
     if not each:
         return wall, ms, n_self
-    _, ms_each = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=steps, each=True)
-    return wall, ms, ms_each, n_self
+    reps = max(200, int(np.ceil(roofline_ms / max(ms, 1e-4))))
+    ms_run = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=reps)
+    _, ms_each, ms_med = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=200, each=True)
+    return wall, ms, n_self, {"reps": reps, "ms_back_to_back": ms_run, "ms_each_mean": ms_each,
+                              "ms_each_median": ms_med}
 
 
 def cpu_baseline(n_test=65536, reps=3):
@@ -98,9 +112,15 @@ def cpu_baseline(n_test=65536, reps=3):
         call()
     dt = (time.perf_counter() - t0) / reps
     n_int = interactions(64, n_test)
-    return {"value": n_int / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
-            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each" % (n_test, reps, dt),
-            "host": host_info()}
+    # The reference's loop visits all N^2 (sink, source) pairs, massless
+    # sources included (common.c:48-59), so its time per evaluation grows as
+    # N^2: extrapolated to the 64 + 1,048,576 workload by that loop count.
+    n_s, n_full = 64 + n_test, 64 + (1 << 20)
+    dt_full = dt * (n_full / n_s) ** 2
+    return {"value": interactions(64, 1 << 20) / dt_full, "unit": "interactions/s", "cores": 1, "kind": kind,
+            "sample": "64 massive + %d test particles, %d evaluations, %.3f s each; value extrapolated to "
+                      "64 + 1,048,576 by the N^2 loop count (%.0f s per evaluation)" % (n_test, reps, dt, dt_full),
+            "sample_value": n_int / dt, "host": host_info()}
 
 
 def cpu_all_cores(n_test=262144, reps=2):
@@ -181,28 +201,35 @@ def gr15_dropin(years, force_mode="fp64", n_test=1000):
     return {"system_yr_per_s": t / wall, "model_yr": t, "wall_s": wall, "dE_E": abs((e1 - e0) / e0)}
 
 
-def ensemble_sample(systems, years, force_mode, controller, proposer=None):
+def ensemble_sample(systems, years, force_mode, controller, proposer=None, max_seconds=120.0):
     """One pass of the batched GR15 engine over an ensemble (state resident on
-    the device), bounded to 120 s; rates use the time each system reached."""
+    the device), bounded to ``max_seconds``; rates use the time each system
+    reached."""
     from abie_amd import engine
     ens = engine.Ensemble(systems, G=G, force_mode=force_mode, controller=controller)
     if proposer is not None:
         ens.set_proposer(proposer)
     e0 = ens.energy()
     t0 = time.perf_counter()
-    done = ens.run(years, max_seconds=120.0)
+    done = ens.run(years, max_seconds=max_seconds)
     wall = time.perf_counter() - t0
     _, _, t, st = ens.state()
     e1 = ens.energy()
     cnt = ens.counters()
+    fin = ens.finish_times()
     ens.close()
     de = np.abs((e1 - e0) / e0)
+    fin_ok = fin[fin >= 0]
+    tail = ({"p50": float(np.median(fin_ok)), "p90": float(np.quantile(fin_ok, 0.9)),
+             "p99": float(np.quantile(fin_ok, 0.99)), "max": float(fin_ok.max())} if fin_ok.size else None)
     return {"n_systems": len(systems), "model_yr_each": years, "done": bool(done), "wall_s": wall,
+            "model_yr_min": float(t.min()), "model_yr_mean": float(t.mean()),
             "system_yr_per_s": float(t.sum() / wall), "evals_per_system_yr": float(cnt[:, 0].sum() / t.sum()),
             "reject_rate": float(1 - cnt[:, 2].sum() / max(cnt[:, 1].sum(), 1)),
             "dE_E_max": float(de.max()), "dE_E_p99": float(np.quantile(de, 0.99)),
             "dE_E_median": float(np.median(de)), "systems_above_1e-10": int((de > 1e-10).sum()),
-            "system_yr_per_s_within_1e-10": float(t[de <= 1e-10].sum() / wall)}
+            "system_yr_per_s_within_1e-10": float(t[de <= 1e-10].sum() / wall),
+            "finish_s": tail}
 
 
 def ensemble_report(n_sys, years, proposer):
@@ -222,6 +249,26 @@ def ensemble_report(n_sys, years, proposer):
         log("ensemble %s: %s" % (name, out[name]))
     base = out["fp64/plain"]["system_yr_per_s"]
     out["speedup_vs_plain"] = {k: v["system_yr_per_s"] / base for k, v in out.items() if k != "fp64/plain"}
+    return out
+
+
+def ensemble_long(n_sys, years, proposer, plain_cap=90.0, chosen_cap=240.0):
+    """The plain FP64 GR15 and the chosen form (learned proposer, mixed test
+    forces) over a longer horizon, where the slowest close-encounter systems
+    set each form's makespan.  The plain form is stopped at ``plain_cap``
+    seconds and its rate counts the model time each system reached."""
+    from abie_amd import ics
+    systems = ics.ensemble(n_sys)
+    out = {"plain": ensemble_sample(systems, years, "fp64", "plain", max_seconds=plain_cap)}
+    log("ensemble %g yr plain: %s" % (years, out["plain"]))
+    if proposer is not None:
+        out["chosen"] = ensemble_sample(systems, years, "mixed", "learned", proposer, max_seconds=chosen_cap)
+        log("ensemble %g yr chosen (mixed/learned): %s" % (years, out["chosen"]))
+        out["ratio"] = out["chosen"]["system_yr_per_s"] / out["plain"]["system_yr_per_s"]
+        out["ratio_within_1e-10"] = (out["chosen"]["system_yr_per_s_within_1e-10"]
+                                     / out["plain"]["system_yr_per_s_within_1e-10"])
+    out["note"] = ("the configuration's 1,000-yr horizon is not measured; the plain form stops at %.0f s and "
+                   "its rate counts the model time each system reached" % plain_cap)
     return out
 
 
@@ -332,6 +379,19 @@ def mixed_long_run(n_test, years, max_seconds=300.0, proposer=None):
     return out
 
 
+def plummer_error_report(pos, m):
+    """Per-acceleration relative difference of the fast FP64 mode from the
+    exact mode (bit-identical to the reference's loop) on every sink."""
+    import torch
+    from abie_amd import engine
+    src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, 1.0)
+    a = torch.stack(engine.accel_soa(src, sinks, n_self, "fp64"), 1)
+    e = torch.stack(engine.accel_soa(src, sinks, n_self, "exact"), 1)
+    rel = (torch.linalg.norm(a - e, dim=1) / torch.linalg.norm(e, dim=1)).cpu().numpy()
+    return {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
+            "p99_rel": float(np.quantile(rel, 0.99)), "frac_within_1e-14": float(np.mean(rel <= 1e-14))}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "force_c1a_traffic.json")
     try:
@@ -348,6 +408,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-long", action="store_true",
+                    help="skip the 10-yr ensemble passes (plain capped at 90 s, chosen form)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the 4,096-system ensemble extras (for kernel traces of the GR15 engine)")
     ap.add_argument("--config2-only", action="store_true",
@@ -405,14 +467,19 @@ def main():
     n_m, n_t = int((m > 0).sum()), int((m == 0).sum())
     n_int = interactions(n_m, n_t)
     log("force workload: %d massive + %d test particles, %d interactions/step" % (n_m, n_t, n_int))
-    wall, ms_event, ms_each, _ = force_case(pos, m, G, "fp64", args.steps, args.warmup, each=True)
+    wall, ms_event, _, rt = force_case(pos, m, G, "fp64", args.steps, args.warmup, each=True)
     value = n_int * args.steps / wall
-    # roofline: algorithmic FLOP over the duration of one launch on its own
-    # (each launch bracketed by its own events: what a per-dispatch kernel
-    # trace measures); back-to-back launches overlap their ramp-up/down
-    achieved = n_int * FLOP_PER_INTERACTION / (ms_each * 1e-3) / 1e12
-    log("fp64: %.3f us/launch back-to-back, %.3f us single launch, %.4g interactions/s"
-        % (ms_event * 1e3, ms_each * 1e3, value))
+    # roofline: algorithmic FLOP over the mean launch duration of a run of
+    # back-to-back launches lasting at least 20 ms (HIP events on the launch
+    # stream around the run; independent of --steps), which is what a
+    # per-dispatch kernel trace averages; single launches bracketed by their
+    # own events also carry the event overhead and are reported beside it
+    ms_launch = rt["ms_back_to_back"]
+    achieved = n_int * FLOP_PER_INTERACTION / (ms_launch * 1e-3) / 1e12
+    log("fp64: %.3f us/launch over %d back-to-back launches (%.3f over the %d timed steps), single launches "
+        "%.3f us mean / %.3f us median, %.4g interactions/s"
+        % (ms_launch * 1e3, rt["reps"], ms_event * 1e3, args.steps, rt["ms_each_mean"] * 1e3,
+           rt["ms_each_median"] * 1e3, value))
 
     out = {
         "metric": METRIC,
@@ -433,7 +500,13 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
                      "kernel": "accel_scalar_kernel<1, FP64>", "flop_per_interaction": FLOP_PER_INTERACTION,
-                     "mean_launch_us": ms_each * 1e3, "mean_launch_us_back_to_back": ms_event * 1e3},
+                     "mean_launch_us": ms_launch * 1e3, "launches_timed": rt["reps"],
+                     "single_launch_us_mean": rt["ms_each_mean"] * 1e3,
+                     "single_launch_us_median": rt["ms_each_median"] * 1e3,
+                     "issue_ceiling_frac": ISSUE_CEILING_FRAC,
+                     "issue_ceiling_note": "16 FP64 VALU + 1 quarter-rate v_rsq_f64 = 20 four-cycle issue "
+                                           "slots per wave per 64 interactions; the 78.6 TFLOP/s peak counts an "
+                                           "FMA (2 FLOP) per slot, i.e. 10 slots for 20 FLOP"},
     }
     extras = {}
     if not args.no_extras:
@@ -443,7 +516,8 @@ def main():
         extras["plummer65536_fp64"] = {"interactions_per_s": interactions(65536, 0) * reps / w,
                                        "mean_launch_us": ms * 1e3,
                                        "frac_fp64_peak": interactions(65536, 0) * 20 / (ms * 1e-3) / 1e12
-                                       / FP64_PEAK_TFLOPS}
+                                       / FP64_PEAK_TFLOPS,
+                                       "vs_exact": plummer_error_report(pp, pm)}
         log("plummer: %s" % extras["plummer65536_fp64"])
         w, ms, _ = force_case(pos, m, G, "mixed", args.steps, args.warmup)
         # the mixed mode's test-particle terms are FP64 instructions (uncorrected
@@ -474,6 +548,8 @@ def main():
         log("gr15 cpu reference: %s" % extras["gr15_config0_cpu_reference_1yr"])
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
         extras["ensemble_config3_4096x0.2yr"] = ensemble_report(4096, 0.2, w)
+        if not args.no_long:
+            extras["ensemble_config3_4096x10yr"] = ensemble_long(4096, 10.0, w)
     if args.configs:
         extras["gr15_dropin_config0_100yr"] = gr15_dropin(100.0)
         log("config 0, 100 yr: %s" % extras["gr15_dropin_config0_100yr"])

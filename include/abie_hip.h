@@ -154,7 +154,10 @@ int abie_hip_set_force_mode(int mode);
  * contributions are summed.  Sinks: SoA px/py/pz; sinks [0, n_self) are the
  * sources [0, n_self) themselves (their self-pair is skipped), sinks beyond
  * are pure sinks.  Writes ax/ay/az[n_sink].  Launched on `stream`
- * (hipStream_t, 0 = default); returns without synchronising.
+ * (hipStream_t, 0 = default); returns without synchronising.  When the sinks
+ * alone cannot fill the GPU the source range is split and the partial sums
+ * go to a stream-ordered allocation (hipMallocAsync on `stream`, freed on it
+ * after the launch), so calls on different streams may overlap.
  * Replaces the kernel of src/gpuforce.cu:60-84 (gpuforce_v2). */
 int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz,
                        const double *gm, int n_src,
@@ -165,14 +168,14 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz,
 /* Times `reps` back-to-back launches of abie_hip_accel_soa on `stream` with
  * HIP events recorded on that stream; *ms_per_launch receives the mean.
  * When ms_each is not NULL a second pass brackets each launch with its own
- * event pair and *ms_each receives the mean single-launch duration (the
- * quantity a per-dispatch kernel trace reports). */
+ * event pair and *ms_each receives the mean single-launch duration, and
+ * *ms_each_median (when not NULL) the median. */
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz,
                             const double *gm, int n_src,
                             const double *px, const double *py, const double *pz,
                             int n_sink, int n_self, double *ax, double *ay, double *az,
                             int mode, void *stream, int reps, double *ms_per_launch,
-                            double *ms_each);
+                            double *ms_each, double *ms_each_median);
 
 /* ---- batched GR15 ensemble: many independent systems, one launch ---- */
 typedef struct abie_ensemble abie_ensemble;
@@ -206,6 +209,11 @@ int abie_ensemble_get_state(abie_ensemble *e, double *pos, double *vel, double *
  * proposer_used, proposer_rejected. */
 int abie_ensemble_get_counters(abie_ensemble *e, long long *counters);
 
+/* Per-system finish times of the last abie_ensemble_run, [n_sys]: seconds of
+ * the device's constant-rate clock from the first start of any unit to the
+ * moment the system's last unit stopped (-1 while a unit is still running). */
+int abie_ensemble_get_finish_times(abie_ensemble *e, double *seconds);
+
 /* Energies (common.c:340-357 per system) of the current state, [n_sys]. */
 int abie_ensemble_energy(abie_ensemble *e, double *energy);
 
@@ -215,8 +223,10 @@ int abie_ensemble_energy(abie_ensemble *e, double *energy);
  * rejections before, previous h / err / dtreq / PC iterations, massive
  * bodies, test particles, next h, and for accepted attempts the largest
  * |v_ij|^2 / r_ij^2 over massive pairs of the new state, and the system's
- * mass ratio (sum of the other massive bodies) / (largest mass). */
-#define ABIE_STEPLOG_FIELDS 18
+ * mass ratio (sum of the other massive bodies) / (largest mass); then
+ * (fields 18..30) the learned proposer's 12 inputs and its raw output as the
+ * kernel evaluated them, NaN when the attempt made no proposal. */
+#define ABIE_STEPLOG_FIELDS 31
 int abie_ensemble_set_steplog(abie_ensemble *e, long long capacity);
 int abie_ensemble_get_steplog(abie_ensemble *e, double *records, long long *n_records);
 
@@ -241,7 +251,8 @@ int abie_gr15_set_tuning(int n_rules, const int *max_bodies, const int *cfg);
 int abie_mlp_forward(const float *w, const float *x, float *y, int n, void *stream);
 
 /* Loss and gradient: mean squared error of y(x) against target[n] with
- * per-sample weights sw[n] (may be NULL); grad[ABIE_MLP_NPARAM], *loss. */
+ * per-sample weights sw[n] (may be NULL); grad[ABIE_MLP_NPARAM], *loss.  Its
+ * workspace is a stream-ordered allocation on `stream`. */
 int abie_mlp_loss_grad(const float *w, const float *x, const float *target, const float *sw,
                        int n, float *grad, float *loss, void *stream);
 

@@ -193,9 +193,11 @@ class Sim:
         p = self.pos if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
         return lib().orc_check_events(ctypes.byref(self.s), _ptr(p), t)
 
-    def accelerations(self, pos=None):
+    def accelerations(self, pos=None, acc=None):
+        """calculate_accelerations into a fresh array, or into a copy of the
+        caller's ``acc`` (whose deleted-body entries the force loop leaves)."""
         p = self.pos if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
-        acc = np.zeros_like(p)
+        acc = np.zeros_like(p) if acc is None else np.ascontiguousarray(acc, dtype=np.float64).reshape(-1).copy()
         rc = lib().orc_calculate_accelerations(ctypes.byref(self.s), _ptr(p), _ptr(self.vel), _ptr(acc))
         if rc != 0:
             raise ValueError("oracle: unsupported configuration (C > 0)")

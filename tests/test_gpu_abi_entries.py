@@ -80,3 +80,24 @@ def test_invalid_device_and_unsupported_calls(lib):
     assert L.initialize_code(G, 0.0, 8, 1, 1, 0) == 0
     assert L.integrator_rk(0.0, 1.0, 0.1) < 0 and L.integrator_wh(0.0, 1.0, 0.1) < 0
     assert b"outside" in L.abie_hip_last_error()
+
+
+def test_integrator_gr_refuses_mixed_forces(lib):
+    """Mixed forces are built for the massive-only controllers only: the
+    reference's controller (every body in its error norms) refuses them with
+    ABIE_HIP_EUNSUP before touching the state, and runs again once the mode
+    is back to fp64; calculate_accelerations keeps honouring the mode."""
+    from abie_amd import _lib, ics
+    pos, vel, m = ics.config0(n_test=40)
+    lib.initialize_code(G, 0.0, len(m), 1, 1, 0.0, deviceID=0)
+    p, v = pos.reshape(-1).copy(), vel.reshape(-1).copy()
+    lib.set_state(p, v, m, np.zeros_like(m), len(m), G, 0.0)
+    assert lib.lib.abie_hip_set_force_mode(2) == 0
+    assert lib.lib.integrator_gr(0.0, 0.01, 1.0) == -5  # ABIE_HIP_EUNSUP
+    assert b"mixed" in _lib.load().abie_hip_last_error()
+    assert lib.get_model_time() == 0.0
+    acc = lib.calculate_accelerations(p, v, len(m), G, m)
+    assert np.all(np.isfinite(acc))
+    assert lib.lib.abie_hip_set_force_mode(0) == 0
+    assert lib.lib.integrator_gr(0.0, 0.01, 1.0) == 0
+    assert lib.get_model_time() > 0.01

@@ -156,6 +156,27 @@ def test_calculate_accelerations_ext_acc(clib, orc):
     clib.set_additional_forces(np.zeros_like(ext))
 
 
+def test_calculate_accelerations_ext_acc_deleted_body(clib, orc):
+    """A deleted body (m < 0) is skipped by the force loop, which leaves the
+    caller's acc entries as they were (common.c:49-52), and then receives the
+    external acceleration like every other body (common.c:107-111)."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(6, 6, 30)
+    m = m.copy()
+    m[3] = -1.0   # a deleted planet
+    m[20] = -1.0  # a deleted test particle
+    ext = np.random.default_rng(3).normal(size=pos.size) * 1e-3
+    init = np.random.default_rng(4).normal(size=pos.size)
+    clib.lib.abie_hip_set_force_mode(1)
+    clib.set_additional_forces(ext)
+    acc = clib.calculate_accelerations(pos.reshape(-1), vel.reshape(-1), len(m), G, m, acc=init.copy())
+    clib.set_additional_forces(np.zeros_like(ext))
+    ref = orc.Sim(pos, vel, m, ext_acc=ext).accelerations(acc=init)
+    assert same_bits(acc.reshape(-1, 3), ref)
+    dead = m < 0
+    assert same_bits(ref[dead], (init.reshape(-1, 3) + ext.reshape(-1, 3))[dead])
+
+
 def test_fp64_full_size_all_sinks_within_1e14_of_exact():
     """Every one of the 1,048,640 sinks of the force-kernel workload: the
     fast FP64 mode against the exact mode (bit-identical to the reference on

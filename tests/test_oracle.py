@@ -278,3 +278,20 @@ def test_threaded_sink_ranges_bit_identical(orc):
     a = orc.accel(pos, G, m)
     b = orc.accel_threads(pos, G, m, 5)
     assert np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def test_calculate_accelerations_deleted_body_keeps_caller_value_plus_ext(orc):
+    """common.c:48-52 skips a deleted sink (m < 0) and common.c:107-111 then
+    adds the external acceleration to all 3N entries: a deleted body ends
+    with the caller's value + ext, and no body feels it as a source."""
+    from abie_amd import ics
+    pos, vel, m = ics.planetary_system(6, 6, 30)
+    m = m.copy()
+    m[3] = -1.0
+    ext = np.random.default_rng(3).normal(size=pos.size) * 1e-3
+    init = np.random.default_rng(4).normal(size=pos.size)
+    a = orc.Sim(pos, vel, m, ext_acc=ext).accelerations(acc=init).reshape(-1)
+    assert np.array_equal(a[9:12], init[9:12] + ext[9:12])
+    keep = np.ones(len(m), bool); keep[3] = False
+    b = orc.Sim(pos[keep], vel[keep], m[keep], ext_acc=ext.reshape(-1, 3)[keep].reshape(-1)).accelerations()
+    assert np.array_equal(a.reshape(-1, 3)[keep], b)
