@@ -53,6 +53,8 @@ SIGNATURES = {
     "abie_hip_set_device": (c_i, [c_i]),
     "abie_hip_set_force_mode": (c_i, [c_i]),
     "abie_hip_accel_soa": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp, c_i, c_vp]),
+    "abie_hip_accel_soa_noalloc": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp,
+                                         c_i, c_vp]),
     "abie_hip_time_accel_soa": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp,
                                       c_i, c_vp, c_i, c_dp, c_dp, c_dp]),
     "abie_ensemble_create": (c_i, [ctypes.POINTER(c_vp), c_i, c_ip, c_dp, c_dp, c_dp, c_dp, c_d, c_i, c_i]),

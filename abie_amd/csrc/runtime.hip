@@ -30,6 +30,9 @@ static int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+// for the entry points of other translation units
+int report_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t _e = (expr);                                                                \

@@ -165,6 +165,21 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz,
                        int n_sink, int n_self, double *ax, double *ay, double *az,
                        int mode, void *stream);
 
+/* abie_hip_accel_soa for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same arguments, kernels and
+ * bits; it calls no hipMalloc, hipFree or allocating library routine.  Its
+ * source-split scratch is a 16 MiB device array of the library, one per GPU,
+ * held until the process ends (a plan needing more returns ABIE_HIP_ETOOBIG;
+ * on a 256-CU MI355X no plan needs more than 12 MiB).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (concurrent streams
+ * or threads) race and must be serialised by the caller. */
+int abie_hip_accel_soa_noalloc(const double *sx, const double *sy, const double *sz,
+                               const double *gm, int n_src,
+                               const double *px, const double *py, const double *pz,
+                               int n_sink, int n_self, double *ax, double *ay, double *az,
+                               int mode, void *stream);
+
 /* Times `reps` back-to-back launches of abie_hip_accel_soa on `stream` with
  * HIP events recorded on that stream; *ms_per_launch receives the mean.
  * When ms_each is not NULL a second pass brackets each launch with its own
