@@ -74,6 +74,43 @@ hipError_t launch_gr15_exact_plain(int, const GR15Args &, int, size_t, hipStream
 hipError_t launch_gr15_mixed_massive(int, const GR15Args &, int, size_t, hipStream_t, bool);
 hipError_t launch_gr15_mixed_learned(int, const GR15Args &, int, size_t, hipStream_t, bool);
 
+hipError_t launch_gr15q_fp64_massive(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15q_fp64_learned(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15q_mixed_massive(int, const GR15Args &, int, size_t, hipStream_t);
+hipError_t launch_gr15q_mixed_learned(int, const GR15Args &, int, size_t, hipStream_t);
+
+int pick_quad(int n) {
+    for (int q = 0; q < kNumQuad; ++q)
+        if (n <= kQuadNT[q] / 4) return q;
+    return -1;
+}
+
+// LDS of the quad form: source lists [2][NT / 4] x {x, y, z, gm}, then the
+// same 128 doubles of scratch as gr15_kernel
+size_t gr15q_lds_bytes(int q) { return (size_t)2 * (kQuadNT[q] / 4) * 4 * sizeof(double) + 128 * sizeof(double); }
+
+hipError_t launch_gr15q(int q, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st) {
+    if (nb <= 0) return hipSuccess;
+    if (q < 0 || q >= kNumQuad) return hipErrorInvalidValue;
+    const size_t lds = gr15q_lds_bytes(q);
+    if (fm == ABIE_FORCE_FP64 && ctrl == ABIE_CTRL_MASSIVE) return launch_gr15q_fp64_massive(q, a, nb, lds, st);
+    if (fm == ABIE_FORCE_FP64 && ctrl == ABIE_CTRL_LEARNED) return launch_gr15q_fp64_learned(q, a, nb, lds, st);
+    if (fm == ABIE_FORCE_MIXED && ctrl == ABIE_CTRL_MASSIVE) return launch_gr15q_mixed_massive(q, a, nb, lds, st);
+    if (fm == ABIE_FORCE_MIXED && ctrl == ABIE_CTRL_LEARNED) return launch_gr15q_mixed_learned(q, a, nb, lds, st);
+    return hipErrorInvalidValue;
+}
+
+__global__ void device_clock_kernel(long long *out) { *out = wall_clock64(); }
+
+hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st) {
+    hipLaunchKernelGGL(device_clock_kernel, dim3(1), dim3(1), 0, st, dev_buf);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(host_tick, dev_buf, sizeof(long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
 bool gr15_combo_supported(int fm, int ctrl) {
     if (fm == ABIE_FORCE_FP64) return ctrl >= 0 && ctrl <= 2;
     if (fm == ABIE_FORCE_FP64_EXACT) return ctrl == ABIE_CTRL_PLAIN;

@@ -80,7 +80,11 @@ struct GR15Args {
     const float *mlp;
     const double *mratio;  // per unit: (sum of the other massive masses) / largest mass
     int max_attempts;  // per launch
-    long long budget_ticks;  // wall-clock ticks a launch may run (checked every 16 attempts); 0: no limit
+    long long budget_ticks;  // wall-clock ticks a group may run (checked every 16 attempts); 0: no limit
+    long long deadline_tick; // device wall clock at which every unit stops (state kept); 0: none
+    int *running;            // units not yet stopped for good in this run; each decrements it once
+    int tail_units;          // one-lane units of the massive-only controllers stop (state kept) once
+                             // *running <= tail_units, to continue in the quad form; 0: never
     int prefetch;            // units with at least this many massive bodies use the prefetching
                              // source loop (massive-only controllers); 0: never
     int events;        // 0: skip the close-encounter / collision scan
@@ -114,6 +118,19 @@ size_t gr15_lds_bytes(int cfg);
 // SIMD target and its LDS): the bound on the slices of one cooperative launch.
 int gr15_blocks_per_cu(int cfg);
 
+// The quad form (gr15_quad.hpp, massive-only controllers): four lanes per
+// body, groups of kQuadNT[q] threads holding kQuadNT[q] / 4 bodies.  A unit
+// in quad group size q carries the configuration index kNumCfg + q.
+constexpr int kNumQuad = 5;
+constexpr int kQuadNT[kNumQuad] = {64, 128, 256, 512, 1024};
+constexpr int kQuadMaxBodies = 256;
+constexpr int kNumAllCfg = kNumCfg + kNumQuad;
+inline bool cfg_is_quad(int cfg) { return cfg >= kNumCfg && cfg < kNumAllCfg; }
+// the smallest quad group size that holds n bodies, or -1
+int pick_quad(int n);
+size_t gr15q_lds_bytes(int q);
+hipError_t launch_gr15q(int q, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st);
+
 bool gr15_combo_supported(int force_mode, int ctrl);
 // coop: a cooperative launch (every block resident at once, or an error),
 // for the slices of one linked system.
@@ -124,6 +141,8 @@ hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
                          int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, int *evflag,
                          hipStream_t st);
+// The device's constant-rate wall clock (wall_clock64) into *host_tick.
+hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st);
 hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,
                          const double *vx, const double *vy, const double *vz, const double *m, double G,
                          double *energy, hipStream_t st);

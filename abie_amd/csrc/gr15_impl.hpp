@@ -85,8 +85,13 @@ constexpr KTab make_ktab() {
 
 static __constant__ KTab gK = make_ktab();
 
-__device__ __forceinline__ const KTab *launder() {
-    const KTab *p = &gK;
+// The table through a constant-address-space pointer: a load through it at a
+// uniform address is selected as a scalar load (s_load into SGPRs); through a
+// generic pointer it would be a vector load whose results occupy VGPRs.
+typedef const __attribute__((address_space(4))) KTab CKTab;
+
+__device__ __forceinline__ CKTab *launder() {
+    CKTab *p = (CKTab *)&gK;
     asm volatile("" : "+s"(p));
     return p;
 }
@@ -99,7 +104,7 @@ __device__ __forceinline__ const KTab *launder() {
 // rounded divisions and separate roundings -- 17 instead of 38 operations
 // per component, within an ulp or two of the reference form.
 template <bool FAST>
-__device__ __forceinline__ double pred_pos(const KTab *K, double y0, double dy0, double a0, const double (&b)[7],
+__device__ __forceinline__ double pred_pos(CKTab *K, double y0, double dy0, double a0, const double (&b)[7],
                                            double s, double T) {
     if (FAST) {
         double p = fma(s * b[6], K->prc[6], b[5] * K->prc[5]);
@@ -123,7 +128,7 @@ __device__ __forceinline__ double pred_pos(const KTab *K, double y0, double dy0,
 
 // approx_vel (integrator_gauss_radau15.c:63-68) at node s = 1.
 template <bool FAST>
-__device__ __forceinline__ double pred_vel1(const KTab *K, double dy0, double a0, const double (&b)[7], double dt) {
+__device__ __forceinline__ double pred_vel1(CKTab *K, double dy0, double a0, const double (&b)[7], double dt) {
     if (FAST) {
         double p = fma(b[6], K->ref2[7], b[5] * K->vrc[5]);
         p = fma(b[4], K->vrc[4], p);
@@ -146,7 +151,7 @@ __device__ __forceinline__ double pred_vel1(const KTab *K, double dy0, double a0
 // compute_gs + compute_bs_from_gs at node IH (1..7) for one component; FAST
 // fuses the b sums.
 template <int IH, bool FAST>
-__device__ __forceinline__ void update_gb(const KTab *K, double F, double a0, double (&g)[7], double (&b)[7]) {
+__device__ __forceinline__ void update_gb(CKTab *K, double F, double a0, double (&g)[7], double (&b)[7]) {
     constexpr int k = IH - 1;
     double v = (F - a0) * K->R[k + 1][0];
 #pragma unroll
@@ -162,7 +167,7 @@ __device__ __forceinline__ void update_gb(const KTab *K, double F, double a0, do
 }
 
 // refine_bs (integrator_gauss_radau15.c:178-218) for one component.
-__device__ __forceinline__ void refine(const KTab *K, double (&b)[7], double q) {
+__device__ __forceinline__ void refine(CKTab *K, double (&b)[7], double q) {
     const double q2 = q * q, q3 = q2 * q, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q2 * q5;
     const double c7 = K->ref[0], c6 = K->ref[1], c5 = K->ref[2], c4 = K->ref[3], c3 = K->ref[4], c2 = K->ref[5];
     const double c21 = K->ref[6], c15 = K->ref[7], c10 = K->ref2[0], c35 = K->ref2[1], c20 = K->ref2[2];
@@ -239,6 +244,79 @@ __device__ __forceinline__ void pair_self(double px, double py, double pz, const
         ax = fma(-t, dx, ax);
         ay = fma(-t, dy, ay);
         az = fma(-t, dz, az);
+    }
+}
+
+// The pair term for a wave that holds massive bodies, without divergence:
+// self removes the self-pair (1.0 added to r^2, where d = 0, so the term is
+// an exact zero), corr applies the FP64 correction of the rsqrt seed.  With
+// self false the bits equal pair<ABIE_FORCE_FP64> (fma(dx, dx, 0) rounds as
+// dx * dx); with corr false they equal pair<ABIE_FORCE_MIXED>.  So a test
+// particle gets the same term whether or not its wave holds massive bodies.
+template <int FM>
+__device__ __forceinline__ void pair_sel(double px, double py, double pz, const double *s, bool self, bool corr,
+                                         double &ax, double &ay, double &az, Coef2 k) {
+    const double dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+    const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, self ? 1.0 : 0.0)));
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double y2 = y * y;
+    const double g3 = (s[3] * y) * y2;
+    const double e = fma(-r2, y2, 1.0);
+    const double tc = fma(g3 * e, fma(e, k.c2, k.c1), g3);
+    const double t = (FM == ABIE_FORCE_FP64 || corr) ? tc : g3;
+    ax = fma(-t, dx, ax);
+    ay = fma(-t, dy, ay);
+    az = fma(-t, dz, az);
+}
+
+// The force sum of the massive-only controllers' units of up to 256 bodies,
+// in both engine forms (one lane per body, gr15_kernel; four lanes per body,
+// gr15q_kernel): the sources are split by j mod 4 into four partial sums
+// P_c over j = c, c + 4, ... in ascending order, combined as
+// (P0 + P1) + (P2 + P3).  Both forms therefore give the same bits, and a
+// unit may move from one to the other between launches.
+//
+// sum_stride4 adds P_c at position p to (ax, ay, az), four sources per
+// iteration in two register sets that alternate: the next two sources are
+// read from LDS while the current two are summed (LDS reads return in
+// order, so each wait covers only the older reads).  The order of the
+// terms, and so the bits, are those of a plain loop.  MW: the wave holds
+// massive bodies (pair_sel with self = (j == me), corr = massive).  The
+// reads ahead are clamped to the last source, so none leaves the list.
+template <int FM, bool MW>
+__device__ __forceinline__ void sum_stride4(const double *S, int c, int nm, int me, bool massive, double px,
+                                            double py, double pz, double &ax, double &ay, double &az, Coef2 kc) {
+    const int cnt = nm > c ? (nm - c + 3) >> 2 : 0;
+    const double *Sp = S + 4 * c;  // source c + 4 k at Sp + 16 k
+    auto term = [&](const double (&q)[4], int k) __attribute__((always_inline)) {
+        if (MW) pair_sel<FM>(px, py, pz, q, c + 4 * k == me, massive, ax, ay, az, kc);
+        else pair<FM>(px, py, pz, q, ax, ay, az, kc);
+    };
+    auto load = [&](double (&q)[4], int k) __attribute__((always_inline)) {
+        const int kk = k < cnt ? k : (cnt > 0 ? cnt - 1 : 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = Sp[16 * kk + u];
+    };
+    double a0[4], a1[4], b0[4], b1[4];
+    load(a0, 0);
+    load(a1, 1);
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        load(b0, k + 2);
+        load(b1, k + 3);
+        term(a0, k);
+        term(a1, k + 1);
+        load(a0, k + 4);
+        load(a1, k + 5);
+        term(b0, k + 2);
+        term(b1, k + 3);
+    }
+    // the remaining 0..3 sources: a0, a1 hold k and k + 1
+    if (k < cnt) term(a0, k);
+    if (k + 1 < cnt) term(a1, k + 1);
+    if (k + 2 < cnt) {
+        load(b0, k + 2);
+        term(b0, k + 2);
     }
 }
 
@@ -449,7 +527,25 @@ struct Engine {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < BPT; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
-        if (CT != ABIE_CTRL_PLAIN && A.prefetch > 0 && nm >= A.prefetch) {
+        if (CT != ABIE_CTRL_PLAIN && BPT == 1) {
+            // the four-way summation tree shared with the quad form
+            const int me = threadIdx.x;
+            const bool mass = me < nm;
+            double s01[3] = {0.0, 0.0, 0.0}, s23[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+            for (int c = 0; c < 4; ++c) {
+                double a[3] = {0.0, 0.0, 0.0};
+                if (wave_massive) sum_stride4<FM, true>(S, c, nm, me, mass, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
+                else sum_stride4<FM, false>(S, c, nm, me, false, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    s01[q] = c == 0 ? a[q] : (c == 1 ? s01[q] + a[q] : s01[q]);
+                    s23[q] = c == 2 ? a[q] : (c == 3 ? s23[q] + a[q] : s23[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[0][q] = s01[q] + s23[q];
+        } else if (CT != ABIE_CTRL_PLAIN && A.prefetch > 0 && nm >= A.prefetch) {
             sum_prefetched(S, p, acc);
         } else if (wave_massive) {
             // slot 0 holds massive bodies (nm <= NT): FP64 with the self-pair removed
@@ -709,8 +805,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         // end the launch once its wall-clock budget is spent (the host then
         // relaunches; the state is stored below either way), so that no
         // unit waits on a fixed attempt count of the slowest one
-        if (A.budget_ticks > 0 && n_att > 0 && (n_att & 15) == 0) {
-            if (tid == 0) flag[2] = (wall_clock64() - t_enter > A.budget_ticks) ? 1.0 : 0.0;
+        if (n_att > 0 && (n_att & 15) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
+            if (tid == 0) {
+                const long long now = wall_clock64();
+                bool stop = (A.budget_ticks > 0 && now - t_enter > A.budget_ticks) ||
+                            (A.deadline_tick > 0 && now > A.deadline_tick);
+                if (!stop && A.tail_units > 0)
+                    stop = __hip_atomic_load(A.running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.tail_units;
+                flag[2] = stop ? 1.0 : 0.0;
+            }
             __syncthreads();
             double stop = flag[2], unused = 0.0;
             if (linked) rmax2(stop, unused);  // every slice stops at the same attempt
@@ -723,13 +826,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             double mdb = -DBL_MAX, mf = -DBL_MAX;
 #define ABIE_NODE(IH)                                                                          \
     {                                                                                          \
-        const KTab *K = launder();                                                             \
+        CKTab *K = launder();                                                             \
         const double s = K->h[IH];                                                             \
         const double Ts = h * s;                                                               \
         _Pragma("unroll") for (int k = 0; k < BPT; ++k) _Pragma("unroll") for (int c = 0; c < 3; ++c) \
             yp[k][c] = pred_pos<kFast>(K, y0[k][c], dy0[k][c], a0[k][c], bb[k][c], s, Ts);   \
         E.forces(yp, acc);                                                                     \
-        const KTab *K2 = launder();                                                            \
+        CKTab *K2 = launder();                                                            \
         _Pragma("unroll") for (int k = 0; k < BPT; ++k) _Pragma("unroll") for (int c = 0; c < 3; ++c) \
             update_gb<IH, kFast>(K2, acc[k][c], a0[k][c], gg[k][c], bb[k][c]);                \
     }
@@ -764,7 +867,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         n_att += 1;
 
         // advance: predictor at s = 1, final evaluation
-        const KTab *KA = launder();
+        CKTab *KA = launder();
 #pragma unroll
         for (int k = 0; k < BPT; ++k)
 #pragma unroll
@@ -928,7 +1031,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             }
             {
                 // also on the last step: the state stays ready to continue
-                const KTab *KR = launder();
+                CKTab *KR = launder();
 #pragma unroll
                 for (int k = 0; k < BPT; ++k)
 #pragma unroll
@@ -1187,7 +1290,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         ctl->n_prop_used += n_pu;
         ctl->n_prop_rej += n_pr;
         if (ctl->start_tick == 0) ctl->start_tick = t_enter;
-        if (status != ST_RUNNING) ctl->done_tick = wall_clock64();
+        if (status != ST_RUNNING) {
+            ctl->done_tick = wall_clock64();
+            if (A.running != nullptr) atomicSub(A.running, 1);
+        }
     }
 }
 

@@ -1,9 +1,12 @@
 // gr15_mixed_learned.hip -- GR15 engine for ABIE_FORCE_MIXED forces under the ABIE_CTRL_LEARNED controller
 // (one translation unit per pair so that they compile in parallel).
-#include "gr15_impl.hpp"
+#include "gr15_quad.hpp"
 
 namespace abie {
 hipError_t launch_gr15_mixed_learned(int cfg, const GR15Args &a, int nb, size_t lds, hipStream_t st, bool coop) {
     return launch_gr15_fm_ct<ABIE_FORCE_MIXED, ABIE_CTRL_LEARNED>(cfg, a, nb, lds, st, coop);
+}
+hipError_t launch_gr15q_mixed_learned(int q, const GR15Args &a, int nb, size_t lds, hipStream_t st) {
+    return launch_gr15q_fm_ct<ABIE_FORCE_MIXED, ABIE_CTRL_LEARNED>(q, a, nb, lds, st);
 }
 }  // namespace abie

@@ -100,10 +100,9 @@ struct abie_ensemble {
     int events = 1;
     double ce_dist = 0.0;
     int max_ce = 0, max_coll = 0;
-    int max_attempts = 1 << 30;       // per launch; the wall-clock budget ends launches
+    int max_attempts = 1 << 30;       // per launch
     int link_above = 1 << 30;         // plain controller: link systems of more bodies than this
     int link_cap = 0;                 // bodies per linked slice (0: the slice size)
-    double launch_seconds = 2.0;      // wall-clock budget of one launch
     long long clock_khz = 0;          // device wall-clock rate
     int n_units = 0;                   // work units (a system, or a slice of one)
     std::vector<int> offsets;          // [n_sys+1] original layout
@@ -111,8 +110,19 @@ struct abie_ensemble {
     std::vector<int> unit_sys, sys_unit0;
     std::vector<SysDesc> desc;         // per unit
     std::vector<int> order;            // grouped by cfg
-    int group_off[kNumCfg + 1] = {0};
-    hipStream_t streams[kNumCfg] = {};
+    // quad form (four lanes per body) of the massive-only controllers: 0 off
+    // (one lane per body throughout), 1 hybrid (one-lane form while the GPU
+    // is full, the quad form once no more units run than there are CUs), 2
+    // the quad form from the start.  Off when the event scan is asked for.
+    int quad = 1;
+    bool tail = false;                 // hybrid: the quad form has taken over
+    std::vector<int> cur_cfg;          // per unit: the form of its next launch
+    std::vector<int> alt_quad;         // per unit: its quad group size, or -1
+    std::vector<int> lpt;              // units that are not linked slices, longest first
+    DBuf<int> d_running;
+    DBuf<long long> d_clock;
+    int group_off[kNumAllCfg + 1] = {0};
+    hipStream_t streams[kNumAllCfg] = {};
     DBuf<SysDesc> d_sys;
     DBuf<int> d_order, d_orig, d_inv;
     DBuf<SysCtl> d_ctl;
@@ -217,6 +227,16 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
                         s, n, nm, max_bodies_one_group(), src_cap(1024));
         }
     }
+    // engine form of a unit that is not a linked slice: the quad form under
+    // the massive-only controllers when the unit fits it, else the
+    // configuration of the tuning rules
+    auto quad_of = [&](int n) {
+        return (e->ctrl != ABIE_CTRL_PLAIN && e->quad != 0 && n <= kQuadMaxBodies) ? pick_quad(n) : -1;
+    };
+    auto unit_cfg = [&](int n, int nm) {
+        const int q = quad_of(n);
+        return (q >= 0 && e->quad == 2) ? kNumCfg + q : pick_cfg(n, nm);
+    };
     const int U = (int)units.size();
     e->n_units = U;
     long long T = 0;
@@ -238,7 +258,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         if (first) e->sys_unit0[s] = u;
         e->unit_sys[u] = s;
         const int n = un.nm + (un.t1 - un.t0);
-        e->desc[u] = SysDesc{(int)g0, n, un.nm, un.cfg >= 0 ? un.cfg : pick_cfg(n, un.nm), (offsets[s + 1] - o) - un.nm,
+        e->desc[u] = SysDesc{(int)g0, n, un.nm, un.cfg >= 0 ? un.cfg : unit_cfg(n, un.nm), (offsets[s + 1] - o) - un.nm,
                               s, first ? 1 : 0, un.nm <= 1 ? 1 : 0, un.link_n, un.link_rank, u - un.link_rank,
                               offsets[s + 1] - o, 0};
         for (int p = 0; p < n; ++p) {
@@ -286,7 +306,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     }
     // LPT order inside each configuration group (largest work first)
     e->order.clear();
-    for (int c = 0; c < kNumCfg; ++c) {
+    for (int c = 0; c < kNumAllCfg; ++c) {
         e->group_off[c] = (int)e->order.size();
         std::vector<int> grp;
         for (int u = 0; u < U; ++u)
@@ -297,7 +317,22 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         });
         e->order.insert(e->order.end(), grp.begin(), grp.end());
     }
-    e->group_off[kNumCfg] = (int)e->order.size();
+    e->group_off[kNumAllCfg] = (int)e->order.size();
+    e->lpt.assign(e->order.begin(), e->order.end());
+    std::stable_sort(e->lpt.begin(), e->lpt.end(), [&](int a, int b) {
+        return (double)e->desc[a].n * e->desc[a].nm > (double)e->desc[b].n * e->desc[b].nm;
+    });
+    e->cur_cfg.resize(U);
+    e->alt_quad.resize(U);
+    for (int u = 0; u < U; ++u) {
+        e->cur_cfg[u] = e->desc[u].cfg;
+        // a unit built with the tree of the quad form can move to it (one-lane
+        // units of one body per thread under the massive-only controllers)
+        const int q = quad_of(e->desc[u].n);
+        e->alt_quad[u] = (q >= 0 && e->desc[u].link_n <= 1 &&
+                          (cfg_is_quad(e->desc[u].cfg) || kCfgs[e->desc[u].cfg].bpt == 1)) ? q : -1;
+    }
+    e->tail = false;
     e->links.clear();
     for (int u = 0; u < U;) {
         const SysDesc &du = e->desc[u];
@@ -368,7 +403,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         c.status = ST_RUNNING;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
-    for (int c = 0; c < kNumCfg; ++c)
+    for (int c = 0; c < kNumAllCfg; ++c)
         if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
     return 0;
 }
@@ -419,7 +454,6 @@ static GR15Args ens_args(abie_ensemble *e) {
     a.mlp = e->has_mlp ? e->d_mlp.p : nullptr;
     a.mratio = e->d_mratio.p;
     a.max_attempts = e->max_attempts;
-    a.budget_ticks = (long long)(e->launch_seconds * 1e3 * (double)e->clock_khz);
     // units of 8+ massive bodies use the prefetching source loop: a lone
     // unit of 20 + 236 bodies 3.07 -> 2.53 us per evaluation, of 12 + 244
     // 2.27 -> 2.01 us, 2,048 of them +2-4 % evaluations per second; same bits
@@ -449,7 +483,10 @@ static int ens_set_t_end(abie_ensemble *e, double t_end, const double *t_start) 
     return 0;
 }
 
-// Launch every configuration group until all systems stop or time is up.
+// Launch the running units, grouped by engine form and longest first, until
+// all systems stop or time is up.  Under the hybrid schedule the one-lane
+// units stop (state kept) once no more units run than there are CUs, and
+// continue in the quad form: the same arithmetic, lower latency per unit.
 static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<SysCtl> ctl(e->n_units);
@@ -460,19 +497,72 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->dev));
         e->clock_khz = khz;
     }
-    if (const char *s = getenv("ABIE_HIP_LAUNCH_SECONDS")) e->launch_seconds = atof(s);
+    // a per-group wall-clock budget only when asked for (tests of relaunches)
+    double group_seconds = 0.0;
+    if (const char *s = getenv("ABIE_HIP_LAUNCH_SECONDS")) group_seconds = atof(s);
+    HIP_TRY(e->d_running.reserve(1));
+    HIP_TRY(e->d_clock.reserve(1));
+    const int n_cu = device_cu_count(e->dev);
+    const bool hybrid = e->ctrl != ABIE_CTRL_PLAIN && e->quad == 1;
+    const int n_plain_units = e->group_off[kNumAllCfg];  // units that are not linked slices
+    if (hybrid) {  // every run starts in the one-lane form (the same bits either way)
+        e->tail = false;
+        for (int u = 0; u < e->n_units; ++u) e->cur_cfg[u] = e->desc[u].cfg;
+    }
+    std::vector<int> ord;
+    ord.reserve(n_plain_units);
     for (;;) {
+        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        int n_run = 0, n_run_units = 0;
+        for (int u = 0; u < e->n_units; ++u)
+            if (ctl[u].status == ST_RUNNING) {
+                ++n_run;
+                if (e->desc[u].link_n <= 1) ++n_run_units;
+            }
+        if (n_run == 0) {
+            done = 1;
+            break;
+        }
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (max_seconds > 0 && el >= max_seconds) break;
+        if (hybrid && !e->tail && n_run_units <= n_cu) {
+            e->tail = true;
+            for (int u = 0; u < e->n_units; ++u)
+                if (e->alt_quad[u] >= 0) e->cur_cfg[u] = kNumCfg + e->alt_quad[u];
+        }
+        int goff[kNumAllCfg + 1];
+        ord.clear();
+        for (int c = 0; c < kNumAllCfg; ++c) {
+            goff[c] = (int)ord.size();
+            for (int u : e->lpt)
+                if (e->cur_cfg[u] == c && ctl[u].status == ST_RUNNING) ord.push_back(u);
+        }
+        goff[kNumAllCfg] = (int)ord.size();
+        if (!ord.empty())
+            HIP_TRY(hipMemcpy(e->d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->d_running.p, &n_run, sizeof(int), hipMemcpyHostToDevice));
         GR15Args a = ens_args(e);
-        for (int c = 0; c < kNumCfg; ++c) {
-            const int nb = e->group_off[c + 1] - e->group_off[c];
+        a.budget_ticks = (long long)(group_seconds * 1e3 * (double)e->clock_khz);
+        a.running = e->d_running.p;
+        a.tail_units = (hybrid && !e->tail) ? n_cu : 0;
+        if (max_seconds > 0) {
+            long long now = 0;
+            HIP_TRY(read_device_clock(&now, e->d_clock.p, e->streams[0]));
+            a.deadline_tick = now + (long long)((max_seconds - el) * 1e3 * (double)e->clock_khz);
+        }
+        for (int c = 0; c < kNumAllCfg; ++c) {
+            const int nb = goff[c + 1] - goff[c];
             if (nb == 0) continue;
             GR15Args ac = a;
-            ac.order = e->d_order.p + e->group_off[c];
-            HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
+            ac.order = e->d_order.p + goff[c];
+            if (cfg_is_quad(c)) HIP_TRY(launch_gr15q(c - kNumCfg, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
+            else HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
         }
-        for (int c = 0; c < kNumCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
+        for (int c = 0; c < kNumAllCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
         // linked systems one at a time, each a cooperative launch of its slices
+        // (their entries of d_order follow the other units' and are never rewritten)
         for (auto &lk : e->links) {
+            if (ctl[e->order[lk[1]]].status != ST_RUNNING) continue;
             GR15Args ac = a;
             ac.order = e->d_order.p + lk[1];
             const hipError_t err = launch_gr15(lk[0], e->force_mode, e->ctrl, ac, lk[2], e->streams[lk[0]], true);
@@ -481,15 +571,6 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
                             "system must be resident together)", lk[2], hipGetErrorString(err));
             HIP_TRY(hipStreamSynchronize(e->streams[lk[0]]));
         }
-        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
-        bool any_running = false;
-        for (auto &c : ctl) any_running |= (c.status == ST_RUNNING);
-        if (!any_running) {
-            done = 1;
-            break;
-        }
-        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (max_seconds > 0 && el >= max_seconds) break;
     }
     if (done_out) *done_out = done;
     return 0;
@@ -520,6 +601,10 @@ int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, con
     e->G = G;
     e->force_mode = force_mode;
     e->ctrl = controller;
+    // the quad form has no close-encounter scan; ABIE_HIP_QUAD=0 turns it off
+    const char *ev_env = getenv("ABIE_HIP_EVENTS");
+    if (const char *q = getenv("ABIE_HIP_QUAD")) e->quad = std::min(2, std::max(0, atoi(q)));
+    if (ev_env && atoi(ev_env) != 0) e->quad = 0;
     int rc = ens_build(e, offsets, pos, vel, m, r, nullptr);
     if (rc == 0) rc = ens_rings(e, 0, 0);
     if (rc != 0) {

@@ -94,12 +94,14 @@ def test_skipped_event_scan_changes_nothing(monkeypatch):
 
 @pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "massive"), ("fp64", "learned")])
 def test_prefetching_source_loop_same_bits(monkeypatch, force_mode, controller):
-    """Units with many massive bodies sum their sources with the prefetching
-    loop under the massive-only controllers; with it on for every unit and
-    off for all, the same bits (odd and even source counts)."""
+    """Units of more than one body per thread (systems of more than 512
+    bodies that cannot be sliced: 211 and 220 massive bodies) sum their sources with the
+    prefetching loop under the massive-only controllers; with it on for
+    every unit and off for all, the same bits (odd and even source
+    counts)."""
     from abie_amd import engine, ics
     systems = [ics.planetary_system(800 + k, nm, nt) for k, (nm, nt) in
-               enumerate([(2, 40), (3, 10), (24, 40), (31, 200), (41, 0), (64, 300), (23, 50)])]
+               enumerate([(220, 400), (211, 500), (2, 40), (31, 200)])]
     w = None
     if controller == "learned":
         import os
