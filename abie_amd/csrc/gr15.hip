@@ -102,6 +102,41 @@ hipError_t launch_gr15q(int q, int fm, int ctrl, const GR15Args &a, int nb, hipS
 
 __global__ void device_clock_kernel(long long *out) { *out = wall_clock64(); }
 
+__global__ void body_gather_kernel(const long long *__restrict__ map, long long n, BodyArrays src, BodyArrays dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = map[i];
+    dst.x[i] = src.x[j]; dst.y[i] = src.y[j]; dst.z[i] = src.z[j];
+    dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j]; dst.vz[i] = src.vz[j];
+    dst.gm[i] = src.gm[j]; dst.m[i] = src.m[j]; dst.rad[i] = src.rad[j];
+    dst.orig[i] = src.orig[j]; dst.inv[i] = src.inv[j]; dst.del[i] = src.del[j];
+    for (int r = 0; r < kStateDoubles; ++r) dst.state[r * dst.total + i] = src.state[r * src.total + j];
+}
+
+__global__ void body_scatter_kernel(const long long *__restrict__ map, long long n, BodyArrays src, BodyArrays dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = map[i];
+    if (j < 0) return;
+    dst.x[j] = src.x[i]; dst.y[j] = src.y[i]; dst.z[j] = src.z[i];
+    dst.vx[j] = src.vx[i]; dst.vy[j] = src.vy[i]; dst.vz[j] = src.vz[i];
+    for (int r = 0; r < kStateDoubles; ++r) dst.state[r * dst.total + j] = src.state[r * src.total + i];
+}
+
+hipError_t launch_body_gather(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
+                              hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(body_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, map, n, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_body_scatter(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
+                               hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(body_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, map, n, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st) {
     hipLaunchKernelGGL(device_clock_kernel, dim3(1), dim3(1), 0, st, dev_buf);
     hipError_t e = hipGetLastError();

@@ -46,6 +46,9 @@ struct SysCtl {
     // launch of the current run and of the moment it stopped running; 0
     // until set (cleared by the host at the start of each run)
     long long start_tick, done_tick;
+    // a unit stops (state kept) once its attempt count reaches this, 0: no
+    // such stop (the host brings the slices of a split unit to one attempt)
+    long long stop_attempt;
 };
 
 enum : int { ST_RUNNING = 0, ST_CE = 1, ST_COLL = 2, ST_DONE = 3, ST_ERROR = -1 };
@@ -83,8 +86,9 @@ struct GR15Args {
     long long budget_ticks;  // wall-clock ticks a group may run (checked every 16 attempts); 0: no limit
     long long deadline_tick; // device wall clock at which every unit stops (state kept); 0: none
     int *running;            // units not yet stopped for good in this run; each decrements it once
-    int tail_units;          // one-lane units of the massive-only controllers stop (state kept) once
-                             // *running <= tail_units, to continue in the quad form; 0: never
+    int tail_units;          // units of the massive-only controllers stop (state kept) once
+                             // *running <= tail_units (the host then moves them to the quad form,
+                             // or splits them finer); 0: never
     int prefetch;            // units with at least this many massive bodies use the prefetching
                              // source loop (massive-only controllers); 0: never
     int events;        // 0: skip the close-encounter / collision scan
@@ -141,6 +145,21 @@ hipError_t launch_gr15(int cfg, int force_mode, int ctrl, const GR15Args &a, int
 hipError_t launch_events(const double *pos, const double *rad, int n, double t, double cd, long long *cnt,
                          int max_ce, int max_coll, double *ring_ce, double *ring_coll, int *flags, int *evflag,
                          hipStream_t st);
+// The per-body arrays of a set of units (the ensemble's, or the split tail's).
+struct BodyArrays {
+    double *x, *y, *z, *vx, *vy, *vz, *gm, *m, *rad, *state;
+    int *orig, *inv;
+    unsigned char *del;
+    long long total;  // bodies; the state is [kStateDoubles][total]
+};
+// dst body i <- src body map[i], every field (i < n).
+hipError_t launch_body_gather(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
+                              hipStream_t st);
+// dst body map[i] <- src body i for positions, velocities and the state,
+// where map[i] >= 0 (i < n).
+hipError_t launch_body_scatter(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
+                               hipStream_t st);
+
 // The device's constant-rate wall clock (wall_clock64) into *host_tick.
 hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st);
 hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,

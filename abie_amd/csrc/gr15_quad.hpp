@@ -76,6 +76,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
     const bool deleted = valid && A.deleted[b0 + bi] != 0;
     const bool in_norm = c < 3 && (d.norm_all != 0 ? valid : massive);
     const bool wave_massive = (tid & ~63) < 4 * nm;
+    // a wave whose 16 bodies all lie beyond the unit's (a unit smaller than
+    // its group, such as a split child given a CU of its own) skips the
+    // force sum and only takes part in the group's barriers
+    const bool wave_idle = (tid & ~63) >= 4 * n;
     const double gm_b = massive ? A.gm[b0 + bi] : 0.0;
     const long long gi = b0 + (valid ? bi : 0);
     Coef2 kc{1.5, 1.875};
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
         // test particles of the same wave as in the other branch (one loop,
         // no divergence)
         if (wave_massive) sum_stride4<FM, true>(S, c, nm, bi, massive, px, py, pz, ax, ay, az, kc);
-        else sum_stride4<FM, false>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
+        else if (!wave_idle) sum_stride4<FM, false>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
         ax = quad_sum(ax);
         ay = quad_sum(ay);
         az = quad_sum(az);
@@ -159,12 +163,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
     }
 
     // ---- the step loop (integrator_gauss_radau15.c:279-368)
-    while (n_att < A.max_attempts) {
-        if (n_att > 0 && (n_att & 15) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0)) {
+    // attempts this launch may make before the unit's attempt count reaches
+    // ctl->stop_attempt (when set)
+    long long stop_at = 0x7fffffffffffffffll;
+    if (ctl->stop_attempt > 0) stop_at = ctl->stop_attempt > ctl->n_attempts ? ctl->stop_attempt - ctl->n_attempts : 0;
+    while (n_att < A.max_attempts && n_att < stop_at) {
+        if (n_att > 0 && (n_att & 15) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
             if (tid == 0) {
                 const long long now = wall_clock64();
-                flag[2] = ((A.budget_ticks > 0 && now - t_enter > A.budget_ticks) ||
-                           (A.deadline_tick > 0 && now > A.deadline_tick)) ? 1.0 : 0.0;
+                bool stop = (A.budget_ticks > 0 && now - t_enter > A.budget_ticks) ||
+                            (A.deadline_tick > 0 && now > A.deadline_tick);
+                if (!stop && A.tail_units > 0)
+                    stop = __hip_atomic_load(A.running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.tail_units;
+                flag[2] = stop ? 1.0 : 0.0;
             }
             __syncthreads();
             if (flag[2] != 0.0) break;

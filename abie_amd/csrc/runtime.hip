@@ -91,6 +91,31 @@ static int engine_order(const double *m, int n, std::vector<int> &perm) {
 using namespace abie;
 
 // ===================================================================== ensemble
+// The split tail of a hybrid run: child units that each carry every massive
+// body of a parent unit and a share of its test particles, in body arrays of
+// their own.  Test particles never act on the massive bodies or on the step
+// under the massive-only controllers, so the children evolve the parent's
+// massive bodies and step sequence exactly, with fewer bodies each: lower
+// latency per evaluation on the CUs that the finished units left idle.
+struct TailSet {
+    int n = 0;                          // child units
+    long long total = 0;                // child bodies
+    std::vector<int> parent;            // per child: parent unit
+    std::vector<SysDesc> desc;          // per child (body0 into these arrays)
+    std::vector<long long> gmap, smap;  // per child body: parent body (gather); parent body or -1 (scatter)
+    DBuf<double> x, y, z, vx, vy, vz, gm, m, rad, state, mratio;
+    DBuf<int> orig, inv, order;
+    DBuf<unsigned char> del;
+    DBuf<SysDesc> sys;
+    DBuf<SysCtl> ctl;
+    DBuf<long long> d_gmap, d_smap;
+    void release() {
+        for (auto *b : {&x, &y, &z, &vx, &vy, &vz, &gm, &m, &rad, &state, &mratio}) b->release();
+        orig.release(); inv.release(); order.release(); del.release(); sys.release(); ctl.release();
+        d_gmap.release(); d_smap.release();
+    }
+};
+
 struct abie_ensemble {
     int dev = 0;
     int n_sys = 0;
@@ -116,11 +141,15 @@ struct abie_ensemble {
     // the quad form from the start.  Off when the event scan is asked for.
     int quad = 1;
     bool tail = false;                 // hybrid: the quad form has taken over
+    bool tail_split_done = false;      // hybrid: the last units run split
     std::vector<int> cur_cfg;          // per unit: the form of its next launch
     std::vector<int> alt_quad;         // per unit: its quad group size, or -1
     std::vector<int> lpt;              // units that are not linked slices, longest first
     DBuf<int> d_running;
     DBuf<long long> d_clock;
+    bool tail_split = true;            // hybrid: split the tail's units over idle CUs
+    TailSet ts;
+    std::vector<double> mratio;        // per unit (host copy of d_mratio)
     int group_off[kNumAllCfg + 1] = {0};
     hipStream_t streams[kNumAllCfg] = {};
     DBuf<SysDesc> d_sys;
@@ -394,6 +423,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         }
         HIP_TRY(e->d_mratio.reserve(U));
         HIP_TRY(hipMemcpy(e->d_mratio.p, mr.data(), U * sizeof(double), hipMemcpyHostToDevice));
+        e->mratio = mr;
     }
     HIP_TRY(hipMemcpy(e->d_order.p, e->order.data(), U * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(e->d_state.p, 0, (size_t)kStateDoubles * T * sizeof(double)));
@@ -483,10 +513,228 @@ static int ens_set_t_end(abie_ensemble *e, double t_end, const double *t_start) 
     return 0;
 }
 
+static BodyArrays main_bodies(abie_ensemble *e) {
+    return BodyArrays{e->d_x.p, e->d_y.p, e->d_z.p, e->d_vx.p, e->d_vy.p, e->d_vz.p, e->d_gm.p, e->d_m.p,
+                      e->d_rad.p, e->d_state.p, e->d_orig.p, e->d_inv.p, e->d_del.p, e->total};
+}
+
+static BodyArrays tail_bodies(TailSet &t) {
+    return BodyArrays{t.x.p, t.y.p, t.z.p, t.vx.p, t.vy.p, t.vz.p, t.gm.p, t.m.p, t.rad.p, t.state.p,
+                      t.orig.p, t.inv.p, t.del.p, t.total};
+}
+
+// The arguments of a launch over the split tail's children.
+static GR15Args tail_args(abie_ensemble *e, const GR15Args &base) {
+    TailSet &t = e->ts;
+    GR15Args a = base;
+    a.sys = t.sys.p;
+    a.ctl = t.ctl.p;
+    a.order = t.order.p;
+    a.x = t.x.p; a.y = t.y.p; a.z = t.z.p;
+    a.vx = t.vx.p; a.vy = t.vy.p; a.vz = t.vz.p;
+    a.gm = t.gm.p;
+    a.rad = t.rad.p;
+    a.ext = nullptr;
+    a.orig = t.orig.p;
+    a.inv = t.inv.p;
+    a.deleted = t.del.p;
+    a.state = t.state.p;
+    a.scan = nullptr;
+    a.total = t.total;
+    a.mratio = t.mratio.p;
+    a.n_units = t.n;
+    return a;
+}
+
+// Split the running units of the quad form over at most n_cu units in all
+// (one per CU, so that each child runs on a CU of its own).  Each extra
+// child goes to the unit with the most remaining work per child -- the
+// attempts still to come, estimated from the unit's progress in this run
+// (rem[u]), times its bodies and massive bodies -- up to the count at which
+// a child fits a 64-body group (more would not shorten its evaluation: each
+// child still sums every massive body).  At least kTailMinTP test particles
+// per child.  Copies the children's bodies out of the ensemble's arrays;
+// split[u] = 1 for the units that now run as children.  Units whose norms
+// cover every body (norm_all) are never split: their step depends on them.
+constexpr int kTailMinTP = 4;
+// the units still running when the split happens (an eighth of the CUs)
+inline int tail_split_units(int n_cu) { return std::max(1, n_cu / 8); }
+static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::vector<double> &rem, int n_cu,
+                   std::vector<char> &split) {
+    TailSet &t = e->ts;
+    t.n = 0;
+    t.total = 0;
+    t.parent.clear(); t.desc.clear(); t.gmap.clear(); t.smap.clear();
+    split.assign(e->n_units, 0);
+    std::vector<int> run;
+    for (int u : e->lpt)
+        if (ctl[u].status == ST_RUNNING) run.push_back(u);
+    int extra = n_cu - (int)run.size();
+    if (run.empty() || extra <= 0) return 0;
+    std::vector<int> kk(e->n_units, 1), cap(e->n_units, 1);
+    std::vector<std::pair<double, int>> heap;  // (work per child, unit)
+    for (int u : run) {
+        const SysDesc &d = e->desc[u];
+        const int nt = d.n - d.nm;
+        if (e->alt_quad[u] < 0 || d.norm_all || d.link_n > 1 || nt < 2 * kTailMinTP) continue;
+        cap[u] = std::max(1, std::min(nt / kTailMinTP, (nt + std::max(kTailMinTP, 64 - d.nm) - 1) /
+                                                           std::max(kTailMinTP, 64 - d.nm)));
+        if (cap[u] > 1) heap.push_back({rem[u] * d.n * std::max(1, d.nm), u});
+    }
+    std::make_heap(heap.begin(), heap.end());
+    while (extra > 0 && !heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end());
+        const int u = heap.back().second;
+        heap.pop_back();
+        kk[u] += 1;
+        --extra;
+        if (kk[u] < cap[u]) {
+            heap.push_back({rem[u] * e->desc[u].n * std::max(1, e->desc[u].nm) / kk[u], u});
+            std::push_heap(heap.begin(), heap.end());
+        }
+    }
+    std::vector<SysCtl> cctl;
+    for (int u : run) {
+        const SysDesc &d = e->desc[u];
+        if (kk[u] < 2) continue;
+        const int nt = d.n - d.nm;
+        const int chunk = (nt + kk[u] - 1) / kk[u];
+        const int k = (nt + chunk - 1) / chunk;
+        split[u] = 1;
+        for (int c = 0; c < k; ++c) {
+            const int t0 = d.nm + c * chunk, t1 = std::min(d.n, t0 + chunk);
+            const int n = d.nm + (t1 - t0);
+            SysDesc cd = d;
+            cd.body0 = (int)t.total;
+            cd.n = n;
+            // the largest group shape: 16 waves at 128 VGPRs fill a CU's
+            // register file, so each child has a CU to itself (the waves
+            // beyond its bodies only join the barriers)
+            cd.cfg = kNumCfg + kNumQuad - 1;
+            cd.primary = (d.primary && c == 0) ? 1 : 0;
+            cd.link_n = 1;
+            cd.link_rank = 0;
+            cd.link_base = t.n;
+            cd.sys_n = n;
+            cd.sys_inv0 = 0;
+            for (int p = 0; p < d.nm; ++p) {
+                t.gmap.push_back(d.body0 + p);
+                t.smap.push_back(c == 0 ? d.body0 + p : -1);  // the massive bodies come back from child 0
+            }
+            for (int p = t0; p < t1; ++p) {
+                t.gmap.push_back(d.body0 + p);
+                t.smap.push_back(d.body0 + p);
+            }
+            t.total += n;
+            t.desc.push_back(cd);
+            t.parent.push_back(u);
+            cctl.push_back(ctl[u]);
+            cctl.back().stop_attempt = 0;
+            ++t.n;
+        }
+    }
+    if (t.n == 0) return 0;
+    const long long T = t.total;
+    for (auto *b : {&t.x, &t.y, &t.z, &t.vx, &t.vy, &t.vz, &t.gm, &t.m, &t.rad}) HIP_TRY(b->reserve(T));
+    HIP_TRY(t.state.reserve((size_t)kStateDoubles * T));
+    HIP_TRY(t.orig.reserve(T)); HIP_TRY(t.inv.reserve(T)); HIP_TRY(t.del.reserve(T));
+    HIP_TRY(t.sys.reserve(t.n)); HIP_TRY(t.ctl.reserve(t.n)); HIP_TRY(t.order.reserve(t.n));
+    HIP_TRY(t.mratio.reserve(t.n));
+    HIP_TRY(t.d_gmap.reserve(T)); HIP_TRY(t.d_smap.reserve(T));
+    std::vector<double> mr(t.n);
+    for (int c = 0; c < t.n; ++c) mr[c] = e->mratio[t.parent[c]];
+    HIP_TRY(hipMemcpy(t.sys.p, t.desc.data(), t.n * sizeof(SysDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.ctl.p, cctl.data(), t.n * sizeof(SysCtl), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.mratio.p, mr.data(), t.n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.d_gmap.p, t.gmap.data(), T * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.d_smap.p, t.smap.data(), T * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(launch_body_gather(t.d_gmap.p, T, main_bodies(e), tail_bodies(t), e->streams[0]));
+    HIP_TRY(hipStreamSynchronize(e->streams[0]));
+    return 0;
+}
+
+// Launch the children listed in `which` (grouped by group size), with
+// arguments `a` over the tail's arrays.
+static int ts_launch(abie_ensemble *e, const GR15Args &a, const std::vector<int> &which) {
+    TailSet &t = e->ts;
+    std::vector<int> ord;
+    int goff[kNumQuad + 1];
+    for (int q = 0; q < kNumQuad; ++q) {
+        goff[q] = (int)ord.size();
+        for (int c : which)
+            if (t.desc[c].cfg == kNumCfg + q) ord.push_back(c);
+    }
+    goff[kNumQuad] = (int)ord.size();
+    if (ord.empty()) return 0;
+    HIP_TRY(hipMemcpy(t.order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice));
+    for (int q = 0; q < kNumQuad; ++q) {
+        const int nb = goff[q + 1] - goff[q];
+        if (nb == 0) continue;
+        GR15Args ac = a;
+        ac.order = t.order.p + goff[q];
+        HIP_TRY(launch_gr15q(q, e->force_mode, e->ctrl, ac, nb, e->streams[kNumCfg + q]));
+    }
+    return 0;
+}
+
+// After a round: bring the children of each parent to one attempt count
+// (a child that stopped earlier runs on to its siblings' count; all of them
+// make the same steps), then copy them back into the ensemble's arrays and
+// the parent's control block, and drop them.
+static int ts_finish(abie_ensemble *e, const GR15Args &base) {
+    TailSet &t = e->ts;
+    if (t.n == 0) return 0;
+    std::vector<SysCtl> cc(t.n);
+    for (int pass = 0;; ++pass) {
+        HIP_TRY(hipMemcpy(cc.data(), t.ctl.p, t.n * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        std::vector<long long> target(e->n_units, -1);
+        for (int c = 0; c < t.n; ++c) target[t.parent[c]] = std::max(target[t.parent[c]], cc[c].n_attempts);
+        std::vector<int> lag;
+        for (int c = 0; c < t.n; ++c)
+            if (cc[c].n_attempts < target[t.parent[c]]) {
+                if (cc[c].status != ST_RUNNING || pass > 0)
+                    return fail(ABIE_HIP_EHIP, "split unit %d: a child stopped at attempt %lld of %lld", t.parent[c],
+                                cc[c].n_attempts, target[t.parent[c]]);
+                cc[c].stop_attempt = target[t.parent[c]];
+                lag.push_back(c);
+            }
+        if (lag.empty()) break;
+        HIP_TRY(hipMemcpy(t.ctl.p, cc.data(), t.n * sizeof(SysCtl), hipMemcpyHostToDevice));
+        GR15Args a = tail_args(e, base);
+        a.deadline_tick = 0;
+        a.budget_ticks = 0;
+        a.tail_units = 0;
+        a.running = nullptr;
+        int rc = ts_launch(e, a, lag);
+        if (rc) return rc;
+        for (int q = 0; q < kNumQuad; ++q) HIP_TRY(hipStreamSynchronize(e->streams[kNumCfg + q]));
+    }
+    HIP_TRY(launch_body_scatter(t.d_smap.p, t.total, tail_bodies(t), main_bodies(e), e->streams[0]));
+    HIP_TRY(hipStreamSynchronize(e->streams[0]));
+    for (int c = 0; c < t.n;) {
+        const int u = t.parent[c];
+        SysCtl pc = cc[c];
+        int c1 = c;
+        for (; c1 < t.n && t.parent[c1] == u; ++c1) {
+            if (cc[c1].start_tick > 0 && (pc.start_tick == 0 || cc[c1].start_tick < pc.start_tick))
+                pc.start_tick = cc[c1].start_tick;
+            pc.done_tick = std::max(pc.done_tick, cc[c1].done_tick);
+        }
+        pc.stop_attempt = 0;
+        HIP_TRY(hipMemcpy(e->d_ctl.p + u, &pc, sizeof(SysCtl), hipMemcpyHostToDevice));
+        c = c1;
+    }
+    t.n = 0;
+    return 0;
+}
+
 // Launch the running units, grouped by engine form and longest first, until
 // all systems stop or time is up.  Under the hybrid schedule the one-lane
 // units stop (state kept) once no more units run than there are CUs, and
 // continue in the quad form: the same arithmetic, lower latency per unit.
+// Once no more than n_cu / 8 units run, those are split over the CUs in
+// proportion to their remaining work (ts_plan); the children run to the end
+// of the run (or its deadline) and are then merged back.
 static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<SysCtl> ctl(e->n_units);
@@ -504,15 +752,28 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     HIP_TRY(e->d_clock.reserve(1));
     const int n_cu = device_cu_count(e->dev);
     const bool hybrid = e->ctrl != ABIE_CTRL_PLAIN && e->quad == 1;
+    const bool splitting = hybrid && e->tail_split && !e->has_ext;
     const int n_plain_units = e->group_off[kNumAllCfg];  // units that are not linked slices
     if (hybrid) {  // every run starts in the one-lane form (the same bits either way)
         e->tail = false;
+        e->tail_split_done = false;
         for (int u = 0; u < e->n_units; ++u) e->cur_cfg[u] = e->desc[u].cfg;
     }
     std::vector<int> ord;
+    std::vector<char> split;
+    std::vector<double> t_run0, rem;          // per unit: model time and attempts at the start of the run
+    std::vector<long long> att_run0;
     ord.reserve(n_plain_units);
-    for (;;) {
+    for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        if (round == 0) {
+            t_run0.resize(e->n_units);
+            att_run0.resize(e->n_units);
+            for (int u = 0; u < e->n_units; ++u) {
+                t_run0[u] = ctl[u].t;
+                att_run0[u] = ctl[u].n_attempts;
+            }
+        }
         int n_run = 0, n_run_units = 0;
         for (int u = 0; u < e->n_units; ++u)
             if (ctl[u].status == ST_RUNNING) {
@@ -525,26 +786,55 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         }
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (max_seconds > 0 && el >= max_seconds) break;
+        split.assign(e->n_units, 0);
         if (hybrid && !e->tail && n_run_units <= n_cu) {
             e->tail = true;
             for (int u = 0; u < e->n_units; ++u)
                 if (e->alt_quad[u] >= 0) e->cur_cfg[u] = kNumCfg + e->alt_quad[u];
+        }
+        if (splitting && e->tail && !e->tail_split_done && n_run_units <= tail_split_units(n_cu)) {
+            e->tail_split_done = true;
+            {
+                // remaining attempts of each unit from its progress so far in
+                // this run (a unit that has not advanced yet counts as the
+                // one with the most progress to make)
+                rem.assign(e->n_units, 0.0);
+                double worst = 1.0;
+                for (int u = 0; u < e->n_units; ++u) {
+                    const double span = ctl[u].t_end - t_run0[u], done_t = ctl[u].t - t_run0[u];
+                    const double att = (double)(ctl[u].n_attempts - att_run0[u]);
+                    if (ctl[u].status == ST_RUNNING && span > 0 && done_t > 0 && att > 0) {
+                        rem[u] = att * std::max(0.0, span - done_t) / done_t;
+                        worst = std::max(worst, rem[u]);
+                    } else {
+                        rem[u] = -1.0;
+                    }
+                }
+                for (auto &r : rem) if (r < 0) r = worst;
+                int rc = ts_plan(e, ctl, rem, n_cu, split);
+                if (rc) return rc;
+            }
         }
         int goff[kNumAllCfg + 1];
         ord.clear();
         for (int c = 0; c < kNumAllCfg; ++c) {
             goff[c] = (int)ord.size();
             for (int u : e->lpt)
-                if (e->cur_cfg[u] == c && ctl[u].status == ST_RUNNING) ord.push_back(u);
+                if (e->cur_cfg[u] == c && ctl[u].status == ST_RUNNING && !split[u]) ord.push_back(u);
         }
         goff[kNumAllCfg] = (int)ord.size();
         if (!ord.empty())
             HIP_TRY(hipMemcpy(e->d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(e->d_running.p, &n_run, sizeof(int), hipMemcpyHostToDevice));
+        const int n_launch = n_run - (n_run_units - (int)ord.size()) + e->ts.n;  // split parents run as children
+        HIP_TRY(hipMemcpy(e->d_running.p, &n_launch, sizeof(int), hipMemcpyHostToDevice));
         GR15Args a = ens_args(e);
         a.budget_ticks = (long long)(group_seconds * 1e3 * (double)e->clock_khz);
         a.running = e->d_running.p;
-        a.tail_units = (hybrid && !e->tail) ? n_cu : 0;
+        // one-lane units yield to the quad form at the tail, quad units to
+        // the split once few remain
+        a.tail_units = !hybrid ? 0
+                       : !e->tail ? n_cu
+                       : (splitting && !e->tail_split_done) ? tail_split_units(n_cu) : 0;
         if (max_seconds > 0) {
             long long now = 0;
             HIP_TRY(read_device_clock(&now, e->d_clock.p, e->streams[0]));
@@ -558,7 +848,17 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             if (cfg_is_quad(c)) HIP_TRY(launch_gr15q(c - kNumCfg, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
             else HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
         }
+        if (e->ts.n > 0) {
+            std::vector<int> all(e->ts.n);
+            std::iota(all.begin(), all.end(), 0);
+            int rc = ts_launch(e, tail_args(e, a), all);
+            if (rc) return rc;
+        }
         for (int c = 0; c < kNumAllCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
+        if (e->ts.n > 0) {
+            int rc = ts_finish(e, a);
+            if (rc) return rc;
+        }
         // linked systems one at a time, each a cooperative launch of its slices
         // (their entries of d_order follow the other units' and are never rewritten)
         for (auto &lk : e->links) {
@@ -604,6 +904,7 @@ int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, con
     // the quad form has no close-encounter scan; ABIE_HIP_QUAD=0 turns it off
     const char *ev_env = getenv("ABIE_HIP_EVENTS");
     if (const char *q = getenv("ABIE_HIP_QUAD")) e->quad = std::min(2, std::max(0, atoi(q)));
+    if (const char *q = getenv("ABIE_HIP_TAIL_SPLIT")) e->tail_split = atoi(q) != 0;
     if (ev_env && atoi(ev_env) != 0) e->quad = 0;
     int rc = ens_build(e, offsets, pos, vel, m, r, nullptr);
     if (rc == 0) rc = ens_rings(e, 0, 0);
@@ -625,6 +926,8 @@ void abie_ensemble_destroy(abie_ensemble *e) {
     e->d_sys.release(); e->d_order.release(); e->d_orig.release(); e->d_inv.release();
     e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release(); e->d_mratio.release();
     e->d_link_count.release(); e->d_link_part.release(); e->d_sys_inv.release();
+    e->d_running.release(); e->d_clock.release();
+    e->ts.release();
     for (auto &s : e->streams)
         if (s) (void)hipStreamDestroy(s);
     delete e;

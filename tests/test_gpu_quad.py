@@ -135,3 +135,30 @@ def test_quad_deleted_body_moves_freely(monkeypatch):
     assert r["done"]
     assert np.array_equal(r["v"][4], vel[4])
     assert np.allclose(r["p"][4], pos[4] + vel[4] * r["t"][0], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "learned")])
+def test_tail_split_same_bits(monkeypatch, force_mode, controller):
+    """The split tail (ABIE_HIP_TAIL_SPLIT=1, the default): once no more than
+    an eighth of the CUs' count of units run -- here from the first launch,
+    these nine systems make about 18 units -- each unit with enough test
+    particles runs as children that carry all its massive bodies and a share
+    of its test particles, on CUs of their own, and is merged back at the
+    end.  Identical bits, times, statuses and counters with the split off,
+    also when the children stop every 7 attempts and catch up with each
+    other before the merge."""
+    import os
+    w = None
+    if controller == "learned":
+        w = np.load(os.path.join(os.path.dirname(__file__), "..", "abie_amd", "data", "proposer.npz"))["weights"]
+    systems = _systems()
+    monkeypatch.setenv("ABIE_HIP_TAIL_SPLIT", "0")
+    ref = _run(systems, 0.2, force_mode, controller, proposer=w)
+    monkeypatch.setenv("ABIE_HIP_TAIL_SPLIT", "1")
+    for chunk in (None, "7"):
+        if chunk:
+            monkeypatch.setenv("ABIE_HIP_MAX_ATTEMPTS", chunk)
+        r = _run(systems, 0.2, force_mode, controller, proposer=w)
+        assert r["done"] and np.array_equal(r["t"], ref["t"]) and np.array_equal(r["st"], ref["st"])
+        assert same_bits(r["p"], ref["p"]) and same_bits(r["v"], ref["v"]), chunk
+        assert np.array_equal(r["cnt"], ref["cnt"]), chunk
