@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libabie_hip.so")
 SOURCES = ["force.hip", "gr15.hip", "gr15_fp64_plain.hip", "gr15_fp64_massive.hip", "gr15_fp64_learned.hip",
            "gr15_exact_plain.hip", "gr15_mixed_massive.hip", "gr15_mixed_learned.hip", "mlp.hip", "noalloc.hip",
-           "runtime.hip"]
+           "clock.hip", "runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ABIE_HIP_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

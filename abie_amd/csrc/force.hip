@@ -289,7 +289,7 @@ __device__ __forceinline__ void scalar_sum(const double *__restrict__ sx, const 
 // Grid (sink blocks, source parts): part p sums sources [p*j_chunk, ...);
 // OUT_PARTIAL writes part[(p*3 + c)*n_sink + i] for reduce_partials.
 template <int SPT, int MODE, bool OUT_PARTIAL>
-__global__ __launch_bounds__(BLOCK) void accel_scalar_kernel(AccelArgs a, int j_chunk, double *__restrict__ part) {
+__device__ __forceinline__ void accel_scalar_body(const AccelArgs &a, int j_chunk, double *__restrict__ part) {
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * BLOCK * SPT;
     const int p = blockIdx.y;
@@ -331,6 +331,56 @@ __global__ __launch_bounds__(BLOCK) void accel_scalar_kernel(AccelArgs a, int j_
             }
         }
     }
+}
+
+template <int SPT, int MODE, bool OUT_PARTIAL>
+__global__ __launch_bounds__(BLOCK) void accel_scalar_kernel(AccelArgs a, int j_chunk, double *__restrict__ part) {
+    accel_scalar_body<SPT, MODE, OUT_PARTIAL>(a, j_chunk, part);
+}
+
+// The one-sink-per-thread scalar-source kernel with each wave's shader cycle
+// counter (s_memtime) and constant-rate wall clock (s_memrealtime) read at
+// its start and end: ticks[4 w ..] = wall0, wall1, cycles0, cycles1.  The
+// clock a VALU-bound kernel runs at depends on its own power draw, so the
+// shader clock for pricing this kernel's instruction stream is read inside it.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void accel_scalar_clocked(AccelArgs a, long long *__restrict__ ticks) {
+    const long long w0 = wall_clock64(), k0 = clock64();
+    accel_scalar_body<1, MODE, false>(a, a.n_src, nullptr);
+    const long long w1 = wall_clock64(), k1 = clock64();
+    if ((threadIdx.x & 63) == 0) {
+        const size_t w = (size_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+        ticks[4 * w + 0] = w0;
+        ticks[4 * w + 1] = w1;
+        ticks[4 * w + 2] = k0;
+        ticks[4 * w + 3] = k1;
+    }
+}
+
+int clocked_waves(const AccelArgs &a, const AccelPlan &pl) {
+    if (!pl.scalar_src || pl.parts != 1 || pl.spt != 1 || a.n_sink <= 0) return 0;
+    return (a.n_sink + BLOCK - 1) / BLOCK * (BLOCK / 64);
+}
+
+hipError_t launch_accel_clocked(const AccelArgs &a, const AccelPlan &pl, long long *ticks, hipStream_t st) {
+    if (clocked_waves(a, pl) == 0) return hipErrorNotSupported;
+    const int nblk = (a.n_sink + BLOCK - 1) / BLOCK;
+    switch (a.mode) {
+        case ABIE_FORCE_FP64:
+            hipLaunchKernelGGL(accel_scalar_clocked<ABIE_FORCE_FP64>, dim3(nblk), dim3(BLOCK), 0, st, a, ticks);
+            break;
+        case ABIE_FORCE_FP64_EXACT:
+            hipLaunchKernelGGL(accel_scalar_clocked<ABIE_FORCE_FP64_EXACT>, dim3(nblk), dim3(BLOCK), 0, st, a, ticks);
+            break;
+        case ABIE_FORCE_MIXED:
+            hipLaunchKernelGGL(accel_scalar_clocked<ABIE_FORCE_MIXED>, dim3(nblk), dim3(BLOCK), 0, st, a, ticks);
+            break;
+        case ABIE_FORCE_FP32:
+            hipLaunchKernelGGL(accel_scalar_clocked<ABIE_FORCE_FP32>, dim3(nblk), dim3(BLOCK), 0, st, a, ticks);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // Sum the j-split partials in part order 0..P-1 (deterministic).

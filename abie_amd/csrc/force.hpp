@@ -28,6 +28,11 @@ inline size_t plan_scratch_doubles(const AccelPlan &pl, int n_sink) {
 }
 // `part` must hold parts*3*n_sink doubles when plan.parts > 1.
 hipError_t launch_accel(const AccelArgs &a, const AccelPlan &plan, double *part, hipStream_t st);
+// The scalar-source unsplit plan (spt 1, parts 1) with per-wave shader-cycle
+// and wall-clock ticks (4 per wave) recorded; clocked_waves: the waves such a
+// launch records, 0 for plans it does not cover.
+int clocked_waves(const AccelArgs &a, const AccelPlan &plan);
+hipError_t launch_accel_clocked(const AccelArgs &a, const AccelPlan &plan, long long *ticks, hipStream_t st);
 
 hipError_t launch_pack(const double *pos, const double *m, const int *perm, int n, int n_src, double G,
                        double *x, double *y, double *z, double *gm, hipStream_t st);

@@ -1500,7 +1500,7 @@ int abie_hip_accel_soa(const double *sx, const double *sy, const double *sz, con
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz, const double *gm, int n_src,
                             const double *px, const double *py, const double *pz, int n_sink, int n_self,
                             double *ax, double *ay, double *az, int mode, void *stream, int reps,
-                            double *ms_per_launch, double *ms_each, double *ms_each_median) {
+                            double *ms_per_launch, double *ms_each, double *ms_each_median, double *shader_ghz) {
     if (reps <= 0 || !ms_per_launch) return fail(ABIE_HIP_EINVAL, "time_accel_soa: bad arguments");
     hipStream_t st = (hipStream_t)stream;
     std::vector<hipEvent_t> ev(ms_each ? 2 * reps : 2);
@@ -1547,6 +1547,49 @@ int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz
         }
     }
     destroy();
+    if (shader_ghz) {
+        // back-to-back launches of the clocked instance of the same kernel;
+        // per launch the median over waves of shader cycles per wall second
+        *shader_ghz = 0.0;
+        int dev = 0, khz = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        AccelArgs a{sx, sy, sz, gm, px, py, pz, ax, ay, az, n_src, n_sink, n_self, mode};
+        const AccelPlan pl = plan_accel(n_src, n_sink, mode, device_cu_count(dev));
+        const int nw = clocked_waves(a, pl);
+        // as many launches as the timed run (the clock settles over the first
+        // milliseconds of a load); the last nl of them keep their ticks
+        const int nl = std::min(reps, 8);
+        if (nw > 0 && khz > 0) {
+            const size_t per = 4 * (size_t)nw;
+            long long *ticks = nullptr;
+            HIP_TRY(hipMalloc((void **)&ticks, per * nl * sizeof(long long)));
+            hipError_t e = hipSuccess;
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                e = launch_accel_clocked(a, pl, ticks + per * (r % nl), st);  // nl consecutive r: distinct slots
+            std::vector<long long> h(per * nl);
+            if (e == hipSuccess) e = hipMemcpyAsync(h.data(), ticks, h.size() * sizeof(long long), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            (void)hipFree(ticks);
+            if (e != hipSuccess) return fail(ABIE_HIP_EHIP, "time_accel_soa: clocked launches: %s", hipGetErrorString(e));
+            std::vector<double> launch_ghz, wave_ghz;
+            for (int r = 0; r < nl; ++r) {
+                wave_ghz.clear();
+                for (int w = 0; w < nw; ++w) {
+                    const long long *t = h.data() + per * r + 4 * w;
+                    const double wall_s = (double)(t[1] - t[0]) / (khz * 1e3);
+                    if (wall_s > 0) wave_ghz.push_back((double)(t[3] - t[2]) / wall_s / 1e9);
+                }
+                if (wave_ghz.empty()) continue;
+                std::nth_element(wave_ghz.begin(), wave_ghz.begin() + wave_ghz.size() / 2, wave_ghz.end());
+                launch_ghz.push_back(wave_ghz[wave_ghz.size() / 2]);
+            }
+            if (!launch_ghz.empty()) {
+                std::sort(launch_ghz.begin(), launch_ghz.end());
+                *shader_ghz = launch_ghz[launch_ghz.size() / 2];
+            }
+        }
+    }
     return 0;
 }
 

@@ -59,25 +59,48 @@ def accel_soa(src, sinks, n_self, mode="fp64", out=None, stream=None):
     return out
 
 
-def time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=100, stream=None, each=False):
+def time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=100, stream=None, each=False, clock=False):
     """Mean milliseconds per launch of ``reps`` back-to-back launches, timed
     with HIP events recorded on the launch stream.  each=True also returns
     the mean and the median duration of a launch bracketed by its own event
-    pair: (back_to_back, each_mean, each_median)."""
+    pair: (back_to_back, each_mean, each_median).  clock=True appends the
+    shader clock (GHz) the kernel ran at, read inside an instance of the same
+    kernel (0.0 for plans other than the unsplit scalar-source kernel)."""
     sx, sy, sz, gm = src
     px, py, pz = sinks
     ax, ay, az = out
     ms = ctypes.c_double(0.0)
     ms1 = ctypes.c_double(0.0)
     med = ctypes.c_double(0.0)
+    ghz = ctypes.c_double(0.0)
     lib = _lib.load()
     _lib.check(lib.abie_hip_time_accel_soa(_ptr(sx), _ptr(sy), _ptr(sz), _ptr(gm), sx.numel(), _ptr(px), _ptr(py),
                                            _ptr(pz), px.numel(), int(n_self), _ptr(ax), _ptr(ay), _ptr(az),
                                            _mode(mode), _stream(stream), int(reps), ctypes.byref(ms),
                                            ctypes.byref(ms1) if each else None,
-                                           ctypes.byref(med) if each else None),
+                                           ctypes.byref(med) if each else None,
+                                           ctypes.byref(ghz) if clock else None),
                "abie_hip_time_accel_soa")
-    return (ms.value, ms1.value, med.value) if each else ms.value
+    r = (ms.value, ms1.value, med.value) if each else (ms.value,)
+    if clock:
+        r = r + (ghz.value,)
+    return r if len(r) > 1 else r[0]
+
+
+ISSUE_KINDS = {"fma": 0, "rsq": 1, "mix": 2}
+
+
+def issue_rate(kind="fma", milliseconds=20.0):
+    """(shader GHz, SIMD cycles per wave instruction) of FP64 VALU work at 8
+    waves per SIMD: kind "fma" (v_fma_f64), "rsq" (v_rsq_f64) or "mix" (one
+    v_rsq_f64 per 16 v_fma_f64), measured by a kernel of about
+    ``milliseconds``."""
+    ghz = ctypes.c_double(0.0)
+    cyc = ctypes.c_double(0.0)
+    lib = _lib.load()
+    _lib.check(lib.abie_hip_issue_rate(ISSUE_KINDS[kind], float(milliseconds), ctypes.byref(ghz),
+                                       ctypes.byref(cyc)), "abie_hip_issue_rate")
+    return ghz.value, cyc.value
 
 
 def soa_from_bodies(pos, m, G, device="cuda"):

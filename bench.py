@@ -46,6 +46,7 @@ FP64_PEAK_TFLOPS = 78.6            # MI355X FP64 vector peak
 # the FP64 interaction issues 16 VALU ops + 1 v_rsq_f64 at quarter rate: 20
 # issue slots against the 10 FMA slots the peak assumes for 20 FLOP
 ISSUE_CEILING_FRAC = 10.0 / 20.0
+PEAK_CLOCK_GHZ = 2.4               # the engine clock the FP64 peak is quoted at
 METRIC = "interactions/s & system-yr/s at |dE/E|<=1e-10, chosen form vs plain FP64 GR15"
 
 
@@ -78,10 +79,42 @@ def force_case(pos, m, Gc, mode, steps, warmup, each=False, roofline_ms=20.0):
     if not each:
         return wall, ms, n_self
     reps = max(200, int(np.ceil(roofline_ms / max(ms, 1e-4))))
-    ms_run = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=reps)
+    # the kernel's clock is read by as many clocked launches right after
+    ms_run, ghz = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=reps, clock=True)
     _, ms_each, ms_med = engine.time_accel_soa(src, sinks, n_self, out, mode=mode, reps=200, each=True)
+    # SIMD cycles per wave instruction of the interaction's instruction kinds
+    # at 8 waves per SIMD: v_fma_f64, v_rsq_f64, and one rsq per 16 FMAs
+    rates = {k: engine.issue_rate(k, roofline_ms) for k in ("fma", "rsq", "mix")}
     return wall, ms, n_self, {"reps": reps, "ms_back_to_back": ms_run, "ms_each_mean": ms_each,
-                              "ms_each_median": ms_med}
+                              "ms_each_median": ms_med, "clock_ghz": ghz, "issue_rates": rates}
+
+
+def force_scan(shifts=(19, 20, 21), roofline_ms=20.0):
+    """The FP64 force kernel on 64 massive + 2^k test particles: per launch
+    (back-to-back, at least ``roofline_ms`` of launches), the shader clock
+    read inside the kernel over as many launches, and the SIMD cycles per
+    wave-source pass at that clock.  A fixed cost per launch shows as more
+    cycles per pass at the smaller sizes."""
+    import torch
+    from abie_amd import engine, ics
+    n_simd = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    mix = engine.issue_rate("mix", roofline_ms)
+    out = {"issue_cycles_per_pass": 17 * mix[1], "rows": []}
+    for k in shifts:
+        pos, vel, m = ics.config1a(n_test=1 << k)
+        src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, G)
+        res = tuple(torch.empty(len(m), dtype=torch.float64, device="cuda") for _ in range(3))
+        ms = engine.time_accel_soa(src, sinks, n_self, res, reps=20)
+        reps = max(100, int(np.ceil(roofline_ms / max(ms, 1e-4))))
+        ms, ghz = engine.time_accel_soa(src, sinks, n_self, res, reps=reps, clock=True)
+        passes = -(-len(m) // 256) * 4 * n_self / n_simd
+        row = {"n_test": 1 << k, "us_per_launch": ms * 1e3, "launches": reps, "kernel_clock_ghz": ghz,
+               "simd_cycles_per_pass": ms * 1e-3 * ghz * 1e9 / passes,
+               "interactions_per_s": interactions(n_self, len(m) - n_self) / (ms * 1e-3)}
+        log("force scan: %s" % row)
+        out["rows"].append(row)
+        del src, sinks, res
+    return out
 
 
 def cpu_baseline(n_test=65536, reps=3):
@@ -412,6 +445,9 @@ def main():
                     help="skip the 10-yr ensemble passes (plain capped at 90 s, chosen form)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the 4,096-system ensemble extras (for kernel traces of the GR15 engine)")
+    ap.add_argument("--force-scan", action="store_true",
+                    help="run only the force kernel at 64 + 2^19, 2^20, 2^21 test particles with the "
+                         "kernel's own clock (cycles per wave-source pass)")
     ap.add_argument("--config2-only", action="store_true",
                     help="run only the 1 star + 4 planets + 100,000 test particle system, 100 yr, "
                          "FP64 / mixed / learned (for kernel traces)")
@@ -440,6 +476,11 @@ def main():
     from abie_amd import _lib, ics
     _lib.load()
     torch.cuda.init()
+
+    if args.force_scan:
+        json_out.write(json.dumps({"force_scan": force_scan()}) + "\n")
+        json_out.flush()
+        return
 
     if args.config2_only:
         w = np.load(args.proposer)["weights"] if os.path.exists(args.proposer) else None
@@ -476,6 +517,23 @@ def main():
     # own events also carry the event overhead and are reported beside it
     ms_launch = rt["ms_back_to_back"]
     achieved = n_int * FLOP_PER_INTERACTION / (ms_launch * 1e-3) / 1e12
+    # The 78.6 TFLOP/s peak is quoted at the 2.4 GHz peak engine clock; under a
+    # full FP64 load the clock settles lower.  The kernel's own clock is read
+    # inside an instance of it (per-wave shader cycles over wall time); with
+    # it, the launch's SIMD cycles per wave-source pass (64 sinks against one
+    # source) compare with the issue cycles of the pass's instructions (16
+    # FP64 VALU + 1 v_rsq_f64, measured live as 17 instructions of the
+    # one-rsq-per-16-FMA kernel).
+    n_simd = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    waves = -(-len(m) // 256) * 4
+    passes = waves * n_m / n_simd
+    cyc_pass = ms_launch * 1e-3 * rt["clock_ghz"] * 1e9 / passes
+    issue_pass = 17 * rt["issue_rates"]["mix"][1]
+    ceil_sus = ISSUE_CEILING_FRAC * rt["clock_ghz"] / PEAK_CLOCK_GHZ
+    log("force kernel at %.3f GHz (read inside it): %.2f SIMD cycles per wave-source pass against %.2f issue "
+        "cycles (FMA %.3f, rsq %.3f cycles per wave instruction): %.3f of the issue ceiling"
+        % (rt["clock_ghz"], cyc_pass, issue_pass, rt["issue_rates"]["fma"][1], rt["issue_rates"]["rsq"][1],
+           issue_pass / cyc_pass))
     log("fp64: %.3f us/launch over %d back-to-back launches (%.3f over the %d timed steps), single launches "
         "%.3f us mean / %.3f us median, %.4g interactions/s"
         % (ms_launch * 1e3, rt["reps"], ms_event * 1e3, args.steps, rt["ms_each_mean"] * 1e3,
@@ -504,6 +562,13 @@ def main():
                      "single_launch_us_mean": rt["ms_each_mean"] * 1e3,
                      "single_launch_us_median": rt["ms_each_median"] * 1e3,
                      "issue_ceiling_frac": ISSUE_CEILING_FRAC,
+                     "kernel_clock_ghz": rt["clock_ghz"],
+                     "issue_ceiling_frac_at_kernel_clock": ceil_sus,
+                     "simd_cycles_per_pass": cyc_pass,
+                     "issue_cycles_per_pass": issue_pass,
+                     "frac_of_issue_ceiling": issue_pass / cyc_pass,
+                     "cycles_per_wave_instruction": {k: {"cycles": v[1], "clock_ghz": v[0]}
+                                                     for k, v in rt["issue_rates"].items()},
                      "issue_ceiling_note": "16 FP64 VALU + 1 quarter-rate v_rsq_f64 = 20 four-cycle issue "
                                            "slots per wave per 64 interactions; the 78.6 TFLOP/s peak counts an "
                                            "FMA (2 FLOP) per slot, i.e. 10 slots for 20 FLOP"},

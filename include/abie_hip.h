@@ -184,13 +184,31 @@ int abie_hip_accel_soa_noalloc(const double *sx, const double *sy, const double 
  * HIP events recorded on that stream; *ms_per_launch receives the mean.
  * When ms_each is not NULL a second pass brackets each launch with its own
  * event pair and *ms_each receives the mean single-launch duration, and
- * *ms_each_median (when not NULL) the median. */
+ * *ms_each_median (when not NULL) the median.  When shader_ghz is not NULL,
+ * `reps` back-to-back launches of an instance of the same kernel that reads
+ * each wave's shader cycle counter and wall clock at its start and end
+ * follow, and *shader_ghz receives the median over the last min(reps, 8)
+ * launches of the median over waves of cycles per wall second: the clock
+ * the kernel settles at under that run (plans of the scalar-source kernel
+ * with one sink per thread and no source split; 0 for other plans). */
 int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz,
                             const double *gm, int n_src,
                             const double *px, const double *py, const double *pz,
                             int n_sink, int n_self, double *ax, double *ay, double *az,
                             int mode, void *stream, int reps, double *ms_per_launch,
-                            double *ms_each, double *ms_each_median);
+                            double *ms_each, double *ms_each_median, double *shader_ghz);
+
+/* Measurement aid, no counterpart in the reference: the shader clock and
+ * the SIMD cycles per wave instruction of FP64 VALU work on the current
+ * device.  A kernel keeps 8 waves per SIMD issuing independent chains of
+ * kind 0 (v_fma_f64), 1 (v_rsq_f64) or 2 (one v_rsq_f64 per 16 v_fma_f64)
+ * for about `milliseconds` (0 < ms <= 10000).  *shader_ghz receives the
+ * median over waves of shader cycles (s_memtime) per second of the
+ * constant-rate wall clock (s_memrealtime); *cycles_per_instruction the
+ * kernel's event-timed duration at that clock over the wave instructions
+ * each SIMD issued.  Allocates and frees its own small buffers; runs on and
+ * synchronises the null stream. */
+int abie_hip_issue_rate(int kind, double milliseconds, double *shader_ghz, double *cycles_per_instruction);
 
 /* ---- batched GR15 ensemble: many independent systems, one launch ---- */
 typedef struct abie_ensemble abie_ensemble;

@@ -101,3 +101,48 @@ def test_integrator_gr_refuses_mixed_forces(lib):
     assert lib.lib.abie_hip_set_force_mode(0) == 0
     assert lib.lib.integrator_gr(0.0, 0.01, 1.0) == 0
     assert lib.get_model_time() > 0.01
+
+
+def test_issue_rate_measurement():
+    """abie_hip_issue_rate at 8 waves per SIMD: a wave64 v_fma_f64 holds a
+    SIMD 4 cycles and v_rsq_f64 (quarter rate) 16, so FMA-only chains run at
+    4-4.5 cycles per instruction, rsq-only at 16-17, and one rsq per 16 FMAs
+    at (16 x 4 + 16) / 17 = 4.7 (+10 %); clocks lie in (0.5, 2.6) GHz; bad
+    arguments are refused."""
+    import ctypes
+    from abie_amd import _lib, engine
+    r = {k: engine.issue_rate(k, 5.0) for k in ("fma", "rsq", "mix")}
+    print(r)
+    for ghz, _ in r.values():
+        assert 0.5 < ghz < 2.6
+    assert 4.0 * 0.98 <= r["fma"][1] <= 4.5
+    assert 16.0 * 0.98 <= r["rsq"][1] <= 17.0
+    assert 80.0 / 17 * 0.98 <= r["mix"][1] <= 80.0 / 17 * 1.1
+    L = _lib.load()
+    a, b = ctypes.c_double(), ctypes.c_double()
+    assert L.abie_hip_issue_rate(0, 0.0, ctypes.byref(a), ctypes.byref(b)) == -1  # ABIE_HIP_EINVAL
+    assert L.abie_hip_issue_rate(3, 5.0, ctypes.byref(a), ctypes.byref(b)) == -1
+    assert L.abie_hip_issue_rate(0, 5.0, None, ctypes.byref(b)) == -1
+
+
+def test_time_accel_soa_reads_the_kernel_clock():
+    """time_accel_soa(clock=True): the clocked instance of the scalar-source
+    kernel writes the same bits as the product kernel and reports a shader
+    clock within (0.5, 2.6) GHz; plans it does not cover report 0."""
+    import torch
+    from abie_amd import engine, ics
+    pos, vel, m = ics.config1a(n_test=1 << 16)
+    src, sinks, n_self, _ = engine.soa_from_bodies(pos, m, G)
+    ref = engine.accel_soa(src, sinks, n_self, "fp64")
+    out = tuple(torch.full_like(ref[0], float("nan")) for _ in range(3))
+    ms, ghz = engine.time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=20, clock=True)
+    torch.cuda.synchronize()
+    print("64 + 65,536: %.2f us per launch at %.3f GHz" % (ms * 1e3, ghz))
+    assert 0.5 < ghz < 2.6
+    for a, b in zip(out, ref):
+        assert torch.equal(a, b)
+    pp, _, pm = ics.plummer(4096)  # 4,096 sources: the LDS-tiled kernel
+    src, sinks, n_self, _ = engine.soa_from_bodies(pp, pm, 1.0)
+    out = engine.accel_soa(src, sinks, n_self, "fp64")
+    _, ghz = engine.time_accel_soa(src, sinks, n_self, out, mode="fp64", reps=5, clock=True)
+    assert ghz == 0.0
