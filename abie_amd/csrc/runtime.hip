@@ -920,6 +920,11 @@ int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, con
 
 void abie_ensemble_destroy(abie_ensemble *e) {
     if (!e) return;
+    // nothing may still run on the streams whose buffers are freed and which
+    // are destroyed below (a cooperative launch of linked slices included)
+    (void)hipSetDevice(e->dev);
+    for (auto &s : e->streams)
+        if (s) (void)hipStreamSynchronize(s);
     for (auto *b : {&e->d_x, &e->d_y, &e->d_z, &e->d_vx, &e->d_vy, &e->d_vz, &e->d_gm, &e->d_m, &e->d_rad,
                     &e->d_ext, &e->d_state, &e->d_scan, &e->d_ring_ce, &e->d_ring_coll, &e->d_log, &e->d_energy})
         b->release();
@@ -1273,6 +1278,10 @@ int initialize_code(double G, double C, size_t N_MAX, size_t MAX_N_CE, size_t MA
 
 int finalize_code(void) {
     gr_cache_drop();
+    if (g.stream) {
+        (void)hipSetDevice(g.dev);
+        (void)hipStreamSynchronize(g.stream);
+    }
     for (auto *b : {&g.d_pos, &g.d_m, &g.d_x, &g.d_y, &g.d_z, &g.d_gm, &g.d_ax, &g.d_ay, &g.d_az, &g.d_acc, &g.d_ext, &g.d_part})
         b->release();
     g.d_perm.release();
