@@ -56,6 +56,7 @@ SIGNATURES = {
     "abie_hip_accel_soa_noalloc": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp,
                                          c_i, c_vp]),
     "abie_hip_issue_rate": (c_i, [c_i, ctypes.c_double, c_vp, c_vp]),
+    "abie_hip_step_root7": (c_d, [c_d]),
     "abie_hip_time_accel_soa": (c_i, [c_vp, c_vp, c_vp, c_vp, c_i, c_vp, c_vp, c_vp, c_i, c_i, c_vp, c_vp, c_vp,
                                       c_i, c_vp, c_i, c_dp, c_dp, c_dp, c_dp]),
     "abie_ensemble_create": (c_i, [ctypes.POINTER(c_vp), c_i, c_ip, c_dp, c_dp, c_dp, c_dp, c_d, c_i, c_i]),

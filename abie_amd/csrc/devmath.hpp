@@ -112,4 +112,31 @@ ABIE_HD double pow_pos(double x, double y) {
     return r.hi + r.lo;
 }
 
+// x^(1/7) for x >= 0, to within an ulp or two, for the step controllers that
+// are not bound to the reference's bits (the massive-only ones).  The binary
+// exponent is split off exactly, x = z 2^(7q) with z in [2^-1, 2^6); an FP32
+// seed for z^(1/7) (relative error ~2^-23) is refined by two Newton steps
+// y <- (6 y + z / y^6) / 7 on y^7 = z, each of which squares the relative
+// error (times 3): 2^-23 -> 2^-44 -> FP64 rounding.  About 40 instructions
+// against the double-double pow_pos's thousands.  0, +inf and NaN map to
+// themselves.
+ABIE_HD double root7(double x) {
+    if (!(x > 0.0) || x > DBL_MAX) return (x < 0.0) ? __builtin_nan("") : x;
+    int e = 0;
+    const double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    int q = e / 7, r = e - 7 * q;   // e = 7 q + r
+    if (r < 0) {
+        r += 7;
+        q -= 1;
+    }
+    const double z = ldexp(m, r);  // [0.5, 64)
+    double y = (double)exp2f(log2f((float)z) * (1.0f / 7.0f));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double y3 = y * y * y;
+        y = (6.0 * y + z / (y3 * y3)) / 7.0;
+    }
+    return ldexp(y, q);
+}
+
 }  // namespace abie

@@ -89,10 +89,10 @@ int pick_quad(int n) {
 // same 128 doubles of scratch as gr15_kernel
 size_t gr15q_lds_bytes(int q) { return (size_t)2 * (kQuadNT[q] / 4) * 4 * sizeof(double) + 128 * sizeof(double); }
 
-hipError_t launch_gr15q(int q, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st) {
+hipError_t launch_gr15q(int q, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st, size_t min_lds) {
     if (nb <= 0) return hipSuccess;
     if (q < 0 || q >= kNumQuad) return hipErrorInvalidValue;
-    const size_t lds = gr15q_lds_bytes(q);
+    const size_t lds = std::max(gr15q_lds_bytes(q) + mlp_lds_bytes(a, ctrl), min_lds);
     if (fm == ABIE_FORCE_FP64 && ctrl == ABIE_CTRL_MASSIVE) return launch_gr15q_fp64_massive(q, a, nb, lds, st);
     if (fm == ABIE_FORCE_FP64 && ctrl == ABIE_CTRL_LEARNED) return launch_gr15q_fp64_learned(q, a, nb, lds, st);
     if (fm == ABIE_FORCE_MIXED && ctrl == ABIE_CTRL_MASSIVE) return launch_gr15q_mixed_massive(q, a, nb, lds, st);
@@ -156,7 +156,7 @@ bool gr15_combo_supported(int fm, int ctrl) {
 hipError_t launch_gr15(int cfg, int fm, int ctrl, const GR15Args &a, int nb, hipStream_t st, bool coop) {
     if (nb <= 0) return hipSuccess;
     if (cfg < 0 || cfg >= kNumCfg) return hipErrorInvalidValue;
-    const size_t lds = gr15_lds_bytes(cfg);
+    const size_t lds = gr15_lds_bytes(cfg) + mlp_lds_bytes(a, ctrl);
     if (fm == ABIE_FORCE_FP64) {
         if (ctrl == ABIE_CTRL_PLAIN) return launch_gr15_fp64_plain(cfg, a, nb, lds, st, coop);
         if (ctrl == ABIE_CTRL_MASSIVE) return launch_gr15_fp64_massive(cfg, a, nb, lds, st, coop);

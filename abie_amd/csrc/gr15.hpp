@@ -133,7 +133,17 @@ inline bool cfg_is_quad(int cfg) { return cfg >= kNumCfg && cfg < kNumAllCfg; }
 // the smallest quad group size that holds n bodies, or -1
 int pick_quad(int n);
 size_t gr15q_lds_bytes(int q);
-hipError_t launch_gr15q(int q, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st);
+hipError_t launch_gr15q(int q, int force_mode, int ctrl, const GR15Args &a, int n_blocks, hipStream_t st,
+                        size_t min_lds = 0);
+
+// The learned proposer's weights are staged in LDS after the 128 doubles of
+// scratch when a kernel of the learned controller has a network: the network
+// then reads them from LDS instead of fetching 20 KB through the caches in
+// rounds of dependent loads once per accepted attempt.
+constexpr size_t kMlpLdsBytes = (((size_t)ABIE_MLP_NPARAM * sizeof(float) + 255) / 256) * 256;
+inline size_t mlp_lds_bytes(const GR15Args &a, int ctrl) {
+    return (ctrl == ABIE_CTRL_LEARNED && a.mlp != nullptr) ? kMlpLdsBytes : 0;
+}
 
 bool gr15_combo_supported(int force_mode, int ctrl);
 // coop: a cooperative launch (every block resident at once, or an error),

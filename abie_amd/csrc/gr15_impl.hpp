@@ -96,6 +96,26 @@ __device__ __forceinline__ CKTab *launder() {
     return p;
 }
 
+// The step-size error err = (estim / epsb)^(1/7) (integrator_gauss_radau15.c:316):
+// the reference's pow, correctly rounded, under the plain controller (its
+// trajectories are the reference's bit for bit); the cheap seventh root
+// (root7, an ulp or two) under the massive-only controllers, which are not
+// bound to the reference's bits and run the same code in every engine form.
+template <int CT>
+__device__ __forceinline__ double step_err(double estim) {
+    return CT == ABIE_CTRL_PLAIN ? pow_pos(estim / kEpsb, 1. / 7) : root7(estim / kEpsb);
+}
+
+// |v_ij|^2 / r_ij^2 of one massive pair, for the proposer's encounter rate
+// (the maximum over the pairs): the hardware reciprocal (v_rcp_f64) instead
+// of a division, since the feature needs a few digits.  Every engine form
+// evaluates a pair with these same operations, so the maximum does not
+// depend on which lane or in which order the pairs are visited.
+__device__ __forceinline__ double pair_rate2(double dx, double dy, double dz, double vx, double vy, double vz) {
+    const double r2 = dx * dx + dy * dy + dz * dz, v2 = vx * vx + vy * vy + vz * vz;
+    return v2 * __builtin_amdgcn_rcp(r2);
+}
+
 // -------------------------------------------------------------- predictors
 // approx_pos (integrator_gauss_radau15.c:56-61) at node s, T = dt*s.
 // approx_pos (integrator_gauss_radau15.c:56-61).  FAST (the massive-only
@@ -321,14 +341,44 @@ __device__ __forceinline__ void sum_stride4(const double *S, int c, int nm, int 
 }
 
 // -------------------------------------------------------------- reductions
-// Workgroup max of two values; NaNs lose (fmax), as in vector_max_abs.
+// v from another lane of the wave by a DPP pattern (both 32-bit halves
+// through v_mov_dpp).  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = (int)(unsigned)(u & 0xffffffffll), hi = (int)(u >> 32);
+    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)rhi << 32) | (long long)(unsigned)rlo);
+}
+
+// v of lane l (a wave-uniform value)
+__device__ __forceinline__ double readlane64(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Max over the 64 lanes of a wave: within each row of 16 lanes by DPP
+// (quad_perm [1,0,3,2] and [2,3,0,1], then row_ror 4 and 8), then over the
+// rows' lanes 0, 16, 32, 48 by v_readlane -- no LDS round trips.  A max is
+// exact, so any order gives the same value; NaNs lose, as with fmax.
+__device__ __forceinline__ double wave_max(double v) {
+    v = fmax(v, dpp64<0xB1>(v));
+    v = fmax(v, dpp64<0x4E>(v));
+    v = fmax(v, dpp64<0x124>(v));
+    v = fmax(v, dpp64<0x128>(v));
+    return fmax(fmax(readlane64(v, 0), readlane64(v, 16)), fmax(readlane64(v, 32), readlane64(v, 48)));
+}
+
+// Workgroup max of two values over the group's first nw waves (all of them
+// unless waves without bodies have ended); NaNs lose (fmax), as in
+// vector_max_abs.
 template <int NT>
-__device__ __forceinline__ void wg_max2(double &a, double &b, double *red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a = fmax(a, __shfl_xor(a, off));
-        b = fmax(b, __shfl_xor(b, off));
-    }
+__device__ __forceinline__ void wg_max2(double &a, double &b, double *red, int nw = NT / 64) {
+    a = wave_max(a);
+    b = wave_max(b);
     const int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) {
@@ -340,8 +390,10 @@ __device__ __forceinline__ void wg_max2(double &a, double &b, double *red) {
     b = red[1];
 #pragma unroll
     for (int i = 1; i < NT / 64; ++i) {
-        a = fmax(a, red[2 * i]);
-        b = fmax(b, red[2 * i + 1]);
+        if (i < nw) {
+            a = fmax(a, red[2 * i]);
+            b = fmax(b, red[2 * i + 1]);
+        }
     }
 }
 
@@ -588,7 +640,8 @@ struct Engine {
 constexpr int kLdsFeat = 96;
 
 __device__ __forceinline__ float mlp_eval(const float *__restrict__ w, const float *x, float *lds_h) {
-    // 12 -> 64 (ReLU) -> 64 (ReLU) -> 1; one wave, lane = hidden unit
+    // 12 -> 64 (ReLU) -> 64 (ReLU) -> 1; one wave, lane = hidden unit; w is
+    // the weights' copy in LDS
     const int lane = threadIdx.x & 63;
     const float *W1 = w, *b1 = W1 + ABIE_MLP_IN * ABIE_MLP_HIDDEN;
     const float *W2 = b1 + ABIE_MLP_HIDDEN, *b2 = W2 + ABIE_MLP_HIDDEN * ABIE_MLP_HIDDEN;
@@ -602,13 +655,67 @@ __device__ __forceinline__ float mlp_eval(const float *__restrict__ w, const flo
     lds_h[lane] = h;
     __builtin_amdgcn_wave_barrier();
     float h2 = b2[lane];
-#pragma unroll 4
+#pragma unroll 8
     for (int i = 0; i < ABIE_MLP_HIDDEN; ++i) h2 = fmaf(lds_h[i], W2[i * ABIE_MLP_HIDDEN + lane], h2);
     h2 = fmaxf(h2, 0.0f);
     float o = h2 * w3[lane];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
     return o + b3[0];
+}
+
+// What the learned proposer's inputs are formed from, after an accepted
+// attempt: this attempt's h, error and PC count, the rejections before it,
+// the previous accepted step's h, error, dtreq and PC count, the encounter
+// rate, the unit's massive bodies, its system's test particles and its mass
+// ratio.
+struct PropIn {
+    double h_att, err, dtreq, prev_h, prev_err, prev_dtreq, rate2, mratio;
+    int pc_used, prev_pc, rej_before, nm;
+    long long nt_sys;
+};
+
+// The proposer's output for an accepted attempt, by the calling (first)
+// wave: lane i < 12 forms input i -- one log10 per lane, the same operations
+// on the same values as the inputs of the training code -- the inputs are
+// broadcast to the wave by v_readlane, and mlp_eval runs the network.
+// Returns the output in every lane and lane i's input in *xi (0 beyond 12).
+// Inputs (all invariant under a change of time unit): log10 h/dtreq_prev,
+// log10 err, log10 err_prev, log10 h/h_prev, pc/12, log10 h sqrt(rate2),
+// log10 N_m, log10(N_t + 1) (clamped to the training range 0..2,048: test
+// particles never enter the massive-only error estimate, so a larger count
+// says nothing more about the step), log10(1 + rejections), log10
+// dtreq/dtreq_prev, pc_prev/12, log10 of the mass ratio.
+__device__ __forceinline__ float proposer_eval(const float *__restrict__ w, const PropIn &f, float *lds_h,
+                                               float *xi) {
+    const int i = threadIdx.x & 63;
+    const double eps = 1e-300;
+    const double pd = f.prev_dtreq;
+    double num, den = 1.0;
+    switch (i) {
+        case 0: num = pd > 0 ? f.h_att : 1.0; den = pd > 0 ? pd : 1.0; break;
+        case 1: num = f.err + eps; break;
+        case 2: num = f.prev_err + eps; break;
+        case 3: num = f.h_att; den = f.prev_h > 0 ? f.prev_h : f.h_att; break;
+        case 5: num = f.h_att * sqrt(f.rate2) + 1e-30; break;
+        case 6: num = (double)f.nm; break;
+        case 7: num = (double)f.nt_sys + 1.0; break;
+        case 8: num = 1.0 + (double)f.rej_before; break;
+        case 9: num = pd > 0 ? f.dtreq : 1.0; den = pd > 0 ? pd : 1.0; break;
+        case 11: num = f.mratio + 1e-12; break;
+        default: num = 1.0; break;
+    }
+    const double l = log10(num / den);  // x / 1.0 is x: the unscaled inputs keep their bits
+    float x = (float)l;
+    if (i == 4) x = (float)f.pc_used / 12.0f;
+    if (i == 7) x = (float)fmin(l, log10(2049.0));
+    if (i == 10) x = (float)f.prev_pc / 12.0f;
+    if (i >= ABIE_MLP_IN) x = 0.0f;
+    *xi = x;
+    float xs[ABIE_MLP_IN];
+#pragma unroll
+    for (int k = 0; k < ABIE_MLP_IN; ++k) xs[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+    return mlp_eval(w, xs, lds_h);
 }
 
 // Close-encounter / collision filter of one body against LDS tile entries
@@ -699,6 +806,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
     const long long T = A.total;
     const int b0 = d.body0, n = d.n, nm = d.nm;
     double *flag = E.red + 40;  // LDS scalars broadcast by thread 0
+    // the proposer's weights to LDS (first read after the first force
+    // evaluation's barrier)
+    const float *mlp_w = nullptr;
+    if (CT == ABIE_CTRL_LEARNED && A.mlp != nullptr) {
+        float *wl = (float *)(E.red + 128);
+        for (int i = tid; i < ABIE_MLP_NPARAM; i += NT) wl[i] = A.mlp[i];
+        mlp_w = wl;
+    }
 
     double y0[BPT][3], dy0[BPT][3], a0[BPT][3], bb[BPT][3][7], gg[BPT][3][7], b6p[BPT][3];
     double yp[BPT][3], acc[BPT][3];
@@ -891,7 +1006,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         // a system without any acceleration (one body alone, no ext-acc)
         // would give 0/0 as in the reference; it moves freely: estim 0
         const double estim = (ma > 0.0) ? mb / ma : 0.0;
-        const double err = pow_pos(estim / kEpsb, 1. / 7);
+        const double err = step_err<CT>(estim);
         const double dtreq = h / err;
         const double h_att = h, t_att = t;
         const int rej_before = rejects;
@@ -941,13 +1056,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
             for (int k = 0; k < BPT; ++k) {
                 const int b = k * NT + tid;
                 if (b >= nm) continue;
+#pragma unroll 4
                 for (int j = 0; j < nm; ++j) {
                     if (j == b) continue;
                     const double dx = y0[k][0] - L[6 * j], dyy = y0[k][1] - L[6 * j + 1], dz = y0[k][2] - L[6 * j + 2];
                     const double vx = dy0[k][0] - L[6 * j + 3], vy = dy0[k][1] - L[6 * j + 4],
                                  vz = dy0[k][2] - L[6 * j + 5];
-                    const double r2 = dx * dx + dyy * dyy + dz * dz, v2 = vx * vx + vy * vy + vz * vz;
-                    best = fmax(best, v2 / r2);
+                    best = fmax(best, pair_rate2(dx, dyy, dz, vx, vy, vz));
                 }
             }
             wg_max2<NT>(best, unused, E.red);
@@ -965,31 +1080,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         // the learned proposer (after accepted steps only)
         if (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr) {
             if (tid < 64) {
-                float x[ABIE_MLP_IN];
-                const double eps = 1e-300;
-                x[0] = (float)log10(ctl->prev_dtreq > 0 ? h_att / ctl->prev_dtreq : 1.0);
-                x[1] = (float)log10(err + eps);
-                x[2] = (float)log10(ctl->prev_err + eps);
-                x[3] = (float)log10(h_att / (ctl->prev_h > 0 ? ctl->prev_h : h_att));
-                x[4] = (float)pc_used / 12.0f;
-                x[5] = (float)log10(h_att * sqrt(rate2) + 1e-30);
-                x[6] = (float)log10((double)nm);
-                // clamped to the training range (0..2,048 test particles): the
-                // test particles never enter the massive-only error estimate,
-                // so a larger count says nothing more about the step
-                x[7] = (float)fmin(log10((double)d.nt_sys + 1.0), log10(2049.0));
-                x[8] = (float)log10(1.0 + (double)rej_before);
-                x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
-                x[10] = (float)ctl->prev_pc / 12.0f;
-                x[11] = (float)log10(A.mratio[sys] + 1e-12);
-                const float yl = mlp_eval(A.mlp, x, (float *)(E.red + 64));
-                if (tid == 0) {
-                    flag[0] = (double)yl;
-                    if (A.log != nullptr) {  // the network's inputs and raw output, for the step log
-#pragma unroll
-                        for (int i = 0; i < ABIE_MLP_IN; ++i) E.red[kLdsFeat + i] = (double)x[i];
-                        E.red[kLdsFeat + ABIE_MLP_IN] = (double)yl;
-                    }
+                const PropIn f{h_att, err, dtreq, ctl->prev_h, ctl->prev_err, ctl->prev_dtreq, rate2, A.mratio[sys],
+                               pc_used, ctl->prev_pc, rej_before, nm, d.nt_sys};
+                float xi;
+                const float yl = proposer_eval(mlp_w, f, (float *)(E.red + 64), &xi);
+                if (tid == 0) flag[0] = (double)yl;
+                if (A.log != nullptr) {  // the network's inputs and raw output, for the step log
+                    if (tid < ABIE_MLP_IN) E.red[kLdsFeat + tid] = (double)xi;
+                    if (tid == 0) E.red[kLdsFeat + ABIE_MLP_IN] = (double)yl;
                 }
             }
             __syncthreads();

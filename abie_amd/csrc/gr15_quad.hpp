@@ -49,10 +49,14 @@ __device__ __forceinline__ double quad_sum(double v) {
     return v + quad_dpp<kQuadSwap2>(v);
 }
 
-// Four waves per SIMD (at most 128 VGPRs): a 1,024-thread group needs it to
-// launch at all, and smaller groups then share a CU four waves deep.
+// Registers: a 1,024-thread group needs four waves per SIMD (at most 128
+// VGPRs) to launch at all; smaller groups serve the tail, one group per CU,
+// where two waves per SIMD (256 VGPRs) leave the state, the force loop's
+// pairs in flight and the step control without spills.
+constexpr int quad_wpe(int nt) { return nt >= 1024 ? 4 : 2; }
 template <int NT, int FM, int CT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr15q_kernel(GR15Args A) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT), quad_wpe(NT)))) void gr15q_kernel(
+    GR15Args A) {
     static_assert(CT != ABIE_CTRL_PLAIN, "the quad form serves the massive-only controllers");
     constexpr int NB = NT / 4;  // bodies per group
     extern __shared__ double lds[];
@@ -64,6 +68,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
     const SysDesc d = A.sys[sys];
     SysCtl *ctl = A.ctl + sys;
     if (ctl->status != ST_RUNNING) return;
+    // Waves that hold none of the unit's bodies end here: the group's
+    // barriers (s_barrier) then wait for the surviving waves only, and a
+    // unit smaller than its group runs at the latency of its own waves.
+    const int nw = max(1, min(NT / 64, (d.n + 15) / 16));  // waves with bodies
+    if ((int)(threadIdx.x >> 6) >= nw) return;
+    // the proposer's weights to LDS (first read after the first force
+    // evaluation's barrier)
+    const float *mlp_w = nullptr;
+    if (CT == ABIE_CTRL_LEARNED && A.mlp != nullptr) {
+        float *wl = (float *)(red + 128);
+        for (int i = threadIdx.x; i < ABIE_MLP_NPARAM; i += 64 * nw) wl[i] = A.mlp[i];
+        mlp_w = wl;
+    }
 
     const int tid = threadIdx.x;
     const int bi = tid >> 2;  // body of this quad
@@ -76,10 +93,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
     const bool deleted = valid && A.deleted[b0 + bi] != 0;
     const bool in_norm = c < 3 && (d.norm_all != 0 ? valid : massive);
     const bool wave_massive = (tid & ~63) < 4 * nm;
-    // a wave whose 16 bodies all lie beyond the unit's (a unit smaller than
-    // its group, such as a split child given a CU of its own) skips the
-    // force sum and only takes part in the group's barriers
-    const bool wave_idle = (tid & ~63) >= 4 * n;
     const double gm_b = massive ? A.gm[b0 + bi] : 0.0;
     const long long gi = b0 + (valid ? bi : 0);
     Coef2 kc{1.5, 1.875};
@@ -122,7 +135,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
         // test particles of the same wave as in the other branch (one loop,
         // no divergence)
         if (wave_massive) sum_stride4<FM, true>(S, c, nm, bi, massive, px, py, pz, ax, ay, az, kc);
-        else if (!wave_idle) sum_stride4<FM, false>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
+        else sum_stride4<FM, false>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
         ax = quad_sum(ax);
         ay = quad_sum(ay);
         az = quad_sum(az);
@@ -144,12 +157,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
         a0 = forces(y0);
         n_evals += 1;
         double d0 = in_norm ? fabs(y0) : -DBL_MAX, d1 = in_norm ? fabs(a0) : -DBL_MAX;
-        wg_max2<NT>(d0, d1, red);
+        wg_max2<NT>(d0, d1, red, nw);
         const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
         const double acc = forces(y0 + dt0 * dy0);
         n_evals += 1;
         double d2 = in_norm ? fabs(acc - a0) : -DBL_MAX, dummy = -DBL_MAX;
-        wg_max2<NT>(d2, dummy, red);
+        wg_max2<NT>(d2, dummy, red, nw);
         d2 = d2 / dt0;
         double dt1;
         if (fmax(d1, d2) <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
@@ -196,7 +209,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
 #undef ABIE_QNODE
             n_evals += 7;
             double mdb = in_norm ? fabs(bb[6] - b6p) : -DBL_MAX, mf = in_norm ? fabs(acc) : -DBL_MAX;
-            wg_max2<NT>(mdb, mf, red);
+            wg_max2<NT>(mdb, mf, red, nw);
             const double pc_err = mdb / mf;
             if (pc_err < kTolpc) break;
             // stop once the correction no longer shrinks (Rein & Spiegel 2015)
@@ -214,9 +227,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
         acc = forces(yp);
         n_evals += 1;
         double mb = in_norm ? fabs(bb[6]) : -DBL_MAX, ma = in_norm ? fabs(acc) : -DBL_MAX;
-        wg_max2<NT>(mb, ma, red);
+        wg_max2<NT>(mb, ma, red, nw);
         const double estim = (ma > 0.0) ? mb / ma : 0.0;
-        const double err = pow_pos(estim / kEpsb, 1. / 7);
+        const double err = step_err<CT>(estim);
         const double dtreq = h / err;
         const double h_att = h, t_att = t;
         const int rej_before = rejects;
@@ -252,15 +265,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
             __syncthreads();
             double best = 0.0, unused = 0.0;
             if (massive) {
+#pragma unroll 4
                 for (int j = c; j < nm; j += 4) {
                     if (j == bi) continue;
                     const double dx = X - L[6 * j], dyy = Y - L[6 * j + 1], dz = Z - L[6 * j + 2];
                     const double vx = VX - L[6 * j + 3], vy = VY - L[6 * j + 4], vz = VZ - L[6 * j + 5];
-                    const double r2 = dx * dx + dyy * dyy + dz * dz, v2 = vx * vx + vy * vy + vz * vz;
-                    best = fmax(best, v2 / r2);
+                    best = fmax(best, pair_rate2(dx, dyy, dz, vx, vy, vz));
                 }
             }
-            wg_max2<NT>(best, unused, red);
+            wg_max2<NT>(best, unused, red, nw);
             rate2 = best;
         }
 
@@ -275,29 +288,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gr
 
         if (CT == ABIE_CTRL_LEARNED && accepted && A.mlp != nullptr) {
             if (tid < 64) {
-                float x[ABIE_MLP_IN];
-                const double eps = 1e-300;
-                x[0] = (float)log10(ctl->prev_dtreq > 0 ? h_att / ctl->prev_dtreq : 1.0);
-                x[1] = (float)log10(err + eps);
-                x[2] = (float)log10(ctl->prev_err + eps);
-                x[3] = (float)log10(h_att / (ctl->prev_h > 0 ? ctl->prev_h : h_att));
-                x[4] = (float)pc_used / 12.0f;
-                x[5] = (float)log10(h_att * sqrt(rate2) + 1e-30);
-                x[6] = (float)log10((double)nm);
-                // clamped to the training range (0..2,048 test particles)
-                x[7] = (float)fmin(log10((double)d.nt_sys + 1.0), log10(2049.0));
-                x[8] = (float)log10(1.0 + (double)rej_before);
-                x[9] = (float)log10(ctl->prev_dtreq > 0 ? dtreq / ctl->prev_dtreq : 1.0);
-                x[10] = (float)ctl->prev_pc / 12.0f;
-                x[11] = (float)log10(A.mratio[sys] + 1e-12);
-                const float yl = mlp_eval(A.mlp, x, (float *)(red + 64));
-                if (tid == 0) {
-                    flag[0] = (double)yl;
-                    if (A.log != nullptr) {
-#pragma unroll
-                        for (int i = 0; i < ABIE_MLP_IN; ++i) red[kLdsFeat + i] = (double)x[i];
-                        red[kLdsFeat + ABIE_MLP_IN] = (double)yl;
-                    }
+                const PropIn f{h_att, err, dtreq, ctl->prev_h, ctl->prev_err, ctl->prev_dtreq, rate2, A.mratio[sys],
+                               pc_used, ctl->prev_pc, rej_before, nm, d.nt_sys};
+                float xi;
+                const float yl = proposer_eval(mlp_w, f, (float *)(red + 64), &xi);
+                if (tid == 0) flag[0] = (double)yl;
+                if (A.log != nullptr) {
+                    if (tid < ABIE_MLP_IN) red[kLdsFeat + tid] = (double)xi;
+                    if (tid == 0) red[kLdsFeat + ABIE_MLP_IN] = (double)yl;
                 }
             }
             __syncthreads();

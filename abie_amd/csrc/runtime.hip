@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "abie_hip.h"
+#include "devmath.hpp"
 #include "force.hpp"
 #include "gr15.hpp"
 
@@ -433,6 +434,11 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         c.status = ST_RUNNING;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
+    // the tail's two streams (quad groups of 512 and 1,024 threads) first:
+    // streams created one after the other get distinct hardware queues
+    const int first[2] = {kNumCfg + 3, kNumCfg + 4};
+    for (int c : first)
+        if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
     for (int c = 0; c < kNumAllCfg; ++c)
         if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
     return 0;
@@ -547,16 +553,26 @@ static GR15Args tail_args(abie_ensemble *e, const GR15Args &base) {
 }
 
 // Split the running units of the quad form over at most n_cu units in all
-// (one per CU, so that each child runs on a CU of its own).  Each extra
-// child goes to the unit with the most remaining work per child -- the
-// attempts still to come, estimated from the unit's progress in this run
-// (rem[u]), times its bodies and massive bodies -- up to the count at which
-// a child fits a 64-body group (more would not shorten its evaluation: each
-// child still sums every massive body).  At least kTailMinTP test particles
-// per child.  Copies the children's bodies out of the ensemble's arrays;
-// split[u] = 1 for the units that now run as children.  Units whose norms
-// cover every body (norm_all) are never split: their step depends on them.
+// (one per CU, so that each child runs on a CU of its own).  Each child
+// carries all of its parent's massive bodies and a share of its test
+// particles (at least kTailMinTP).  A unit's evaluation latency steps with
+// the wave rows each SIMD runs, ceil(n / 64) for n bodies at four lanes per
+// body, so a split only pays where its children need fewer rows than the
+// parent: the unit with the most remaining time -- attempts still to come
+// (rem[u], from its progress in this run) times its rows times (16 + its
+// massive bodies) -- repeatedly moves to the next lower row count while the
+// idle CUs allow it.  Every unit that can run in the quad form moves to the
+// tail's arrays, as children or whole, so that the rest of the run is one
+// launch per tail group size; split[u] = 1 for those units.  Units whose
+// norms cover every body (norm_all) move whole: their step depends on every
+// body.
 constexpr int kTailMinTP = 4;
+// The quad group a unit of n bodies runs in at the tail: 512 threads up to
+// 128 bodies, 1,024 beyond (waves without bodies end at the kernel's start).
+// Two group sizes at most, so the tail is at most two concurrent launches,
+// on the two streams created first: launches on streams that share a
+// hardware queue run one after the other.
+inline int tail_q(int n) { return n <= kQuadNT[3] / 4 ? 3 : 4; }
 // the units still running when the split happens (an eighth of the CUs)
 inline int tail_split_units(int n_cu) { return std::max(1, n_cu / 8); }
 static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::vector<double> &rem, int n_cu,
@@ -571,35 +587,48 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
         if (ctl[u].status == ST_RUNNING) run.push_back(u);
     int extra = n_cu - (int)run.size();
     if (run.empty() || extra <= 0) return 0;
-    std::vector<int> kk(e->n_units, 1), cap(e->n_units, 1);
-    std::vector<std::pair<double, int>> heap;  // (work per child, unit)
+    auto rows = [](int n) { return (n + 63) / 64; };
+    std::vector<int> kk(e->n_units, 1);
+    std::vector<int> cand;
     for (int u : run) {
         const SysDesc &d = e->desc[u];
-        const int nt = d.n - d.nm;
-        if (e->alt_quad[u] < 0 || d.norm_all || d.link_n > 1 || nt < 2 * kTailMinTP) continue;
-        cap[u] = std::max(1, std::min(nt / kTailMinTP, (nt + std::max(kTailMinTP, 64 - d.nm) - 1) /
-                                                           std::max(kTailMinTP, 64 - d.nm)));
-        if (cap[u] > 1) heap.push_back({rem[u] * d.n * std::max(1, d.nm), u});
+        if (e->alt_quad[u] >= 0 && !d.norm_all && d.link_n <= 1 && d.n - d.nm >= 2 * kTailMinTP && rows(d.n) > 1)
+            cand.push_back(u);
     }
-    std::make_heap(heap.begin(), heap.end());
-    while (extra > 0 && !heap.empty()) {
-        std::pop_heap(heap.begin(), heap.end());
-        const int u = heap.back().second;
-        heap.pop_back();
-        kk[u] += 1;
-        --extra;
-        if (kk[u] < cap[u]) {
-            heap.push_back({rem[u] * e->desc[u].n * std::max(1, e->desc[u].nm) / kk[u], u});
-            std::push_heap(heap.begin(), heap.end());
+    while (extra > 0 && !cand.empty()) {
+        int bi = 0;
+        double bt = -1.0;
+        for (int i = 0; i < (int)cand.size(); ++i) {
+            const SysDesc &d = e->desc[cand[i]];
+            const int nt = d.n - d.nm, chunk = (nt + kk[cand[i]] - 1) / kk[cand[i]];
+            const double tt = rem[cand[i]] * rows(d.nm + chunk) * (16.0 + d.nm);
+            if (tt > bt) {
+                bt = tt;
+                bi = i;
+            }
         }
+        const int u = cand[bi];
+        const SysDesc &d = e->desc[u];
+        const int nt = d.n - d.nm;
+        const int r = rows(d.nm + (nt + kk[u] - 1) / kk[u]) - 1;  // the next lower row count
+        const int room = 64 * r - d.nm;                              // test particles per child there
+        const int want = room >= kTailMinTP ? (nt + room - 1) / room : 0;
+        if (r < 1 || want == 0 || want - kk[u] > extra) {
+            cand.erase(cand.begin() + bi);  // no lower row count within reach
+            continue;
+        }
+        extra -= want - kk[u];
+        kk[u] = want;
     }
+    // every unit that can run in the quad form moves to the tail's arrays,
+    // split or whole (kk = 1), so that the tail is one launch per group size
     std::vector<SysCtl> cctl;
     for (int u : run) {
         const SysDesc &d = e->desc[u];
-        if (kk[u] < 2) continue;
+        if (e->alt_quad[u] < 0) continue;
         const int nt = d.n - d.nm;
-        const int chunk = (nt + kk[u] - 1) / kk[u];
-        const int k = (nt + chunk - 1) / chunk;
+        const int chunk = nt > 0 ? (nt + kk[u] - 1) / kk[u] : 0;
+        const int k = nt > 0 ? (nt + chunk - 1) / chunk : 1;
         split[u] = 1;
         for (int c = 0; c < k; ++c) {
             const int t0 = d.nm + c * chunk, t1 = std::min(d.n, t0 + chunk);
@@ -607,10 +636,7 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
             SysDesc cd = d;
             cd.body0 = (int)t.total;
             cd.n = n;
-            // the largest group shape: 16 waves at 128 VGPRs fill a CU's
-            // register file, so each child has a CU to itself (the waves
-            // beyond its bodies only join the barriers)
-            cd.cfg = kNumCfg + kNumQuad - 1;
+            cd.cfg = kNumCfg + tail_q(n);
             cd.primary = (d.primary && c == 0) ? 1 : 0;
             cd.link_n = 1;
             cd.link_rank = 0;
@@ -790,7 +816,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         if (hybrid && !e->tail && n_run_units <= n_cu) {
             e->tail = true;
             for (int u = 0; u < e->n_units; ++u)
-                if (e->alt_quad[u] >= 0) e->cur_cfg[u] = kNumCfg + e->alt_quad[u];
+                if (e->alt_quad[u] >= 0) e->cur_cfg[u] = kNumCfg + tail_q(e->desc[u].n);
         }
         if (splitting && e->tail && !e->tail_split_done && n_run_units <= tail_split_units(n_cu)) {
             e->tail_split_done = true;
@@ -1451,6 +1477,8 @@ int abie_gr15_set_tuning(int n_rules, const int *max_bodies, const int *cfg) {
     if (set_tuning(n_rules, max_bodies, cfg) != 0) return fail(ABIE_HIP_EINVAL, "bad tuning rules");
     return 0;
 }
+
+double abie_hip_step_root7(double x) { return abie::root7(x); }
 
 int abie_hip_set_force_mode(int mode) {
     if (mode < 0 || mode > 2) return fail(ABIE_HIP_EINVAL, "bad force mode %d", mode);

@@ -210,6 +210,13 @@ int abie_hip_time_accel_soa(const double *sx, const double *sy, const double *sz
  * synchronises the null stream. */
 int abie_hip_issue_rate(int kind, double milliseconds, double *shader_ghz, double *cycles_per_instruction);
 
+/* Check aid, no counterpart in the reference: the seventh root that the
+ * massive-only step controllers take of estim / epsb (the reference's
+ * pow(x, 1./7), integrator_gauss_radau15.c:316, correctly rounded, stays
+ * with the plain controller), evaluated on the host with the same code as
+ * on the device.  x >= 0; returns x for 0, +inf and NaN. */
+double abie_hip_step_root7(double x);
+
 /* ---- batched GR15 ensemble: many independent systems, one launch ---- */
 typedef struct abie_ensemble abie_ensemble;
 

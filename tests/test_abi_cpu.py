@@ -40,3 +40,33 @@ def test_binding_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.AbieHipError):
         _lib.load()
+
+
+def test_step_root7_within_two_ulp():
+    """The seventh root that the massive-only step controllers take of
+    estim / epsb (abie_hip_step_root7, the device code evaluated on the
+    host): within 2 ulp of the exact root (mpmath, 40 digits) over the whole
+    double range, exact on 2^(7k) and 1, and 0, +inf, NaN map to themselves."""
+    import math
+    import mpmath as mp
+    import numpy as np
+    from abie_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libabie_hip.so not built (run python -m abie_amd.build)")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    f = lib.abie_hip_step_root7
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double]
+    mp.mp.dps = 40
+    rng = np.random.default_rng(7)
+    xs = list(10.0 ** rng.uniform(-307, 307, 400)) + list(10.0 ** rng.uniform(-12, 4, 400))
+    xs += [5e-324, 2.2250738585072014e-308, 1.7976931348623157e308, 1e-4, 3.0, 128.0 - 2 ** -45]
+    worst = 0.0
+    for x in xs:
+        y = f(x)
+        exact = mp.root(mp.mpf(x), 7)
+        worst = max(worst, float(abs(mp.mpf(y) - exact) / mp.mpf(math.ulp(float(exact)))))
+    assert worst <= 2.0, worst
+    for k in range(-40, 41):
+        assert f(2.0 ** (7 * k)) == 2.0 ** k
+    assert f(1.0) == 1.0 and f(0.0) == 0.0 and f(math.inf) == math.inf and math.isnan(f(math.nan))
