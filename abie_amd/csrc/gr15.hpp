@@ -83,7 +83,7 @@ struct GR15Args {
     const float *mlp;
     const double *mratio;  // per unit: (sum of the other massive masses) / largest mass
     int max_attempts;  // per launch
-    long long budget_ticks;  // wall-clock ticks a group may run (checked every 16 attempts); 0: no limit
+    long long budget_ticks;  // wall-clock ticks a group may run (checked every 4 attempts); 0: no limit
     long long deadline_tick; // device wall clock at which every unit stops (state kept); 0: none
     int *running;            // units not yet stopped for good in this run; each decrements it once
     int tail_units;          // units of the massive-only controllers stop (state kept) once

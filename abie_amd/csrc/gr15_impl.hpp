@@ -920,7 +920,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         // end the launch once its wall-clock budget is spent (the host then
         // relaunches; the state is stored below either way), so that no
         // unit waits on a fixed attempt count of the slowest one
-        if (n_att > 0 && (n_att & 15) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
+        // (every 4 attempts; every 16 for linked slices, whose stop is a grid barrier)
+        if (n_att > 0 && (n_att & (linked ? 15 : 3)) == 0 &&
+            (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
             if (tid == 0) {
                 const long long now = wall_clock64();
                 bool stop = (A.budget_ticks > 0 && now - t_enter > A.budget_ticks) ||

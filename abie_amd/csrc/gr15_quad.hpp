@@ -181,7 +181,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
     long long stop_at = 0x7fffffffffffffffll;
     if (ctl->stop_attempt > 0) stop_at = ctl->stop_attempt > ctl->n_attempts ? ctl->stop_attempt - ctl->n_attempts : 0;
     while (n_att < A.max_attempts && n_att < stop_at) {
-        if (n_att > 0 && (n_att & 15) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
+        if (n_att > 0 && (n_att & 3) == 0 && (A.budget_ticks > 0 || A.deadline_tick > 0 || A.tail_units > 0)) {
             if (tid == 0) {
                 const long long now = wall_clock64();
                 bool stop = (A.budget_ticks > 0 && now - t_enter > A.budget_ticks) ||

@@ -149,6 +149,7 @@ struct abie_ensemble {
     DBuf<int> d_running;
     DBuf<long long> d_clock;
     bool tail_split = true;            // hybrid: split the tail's units over idle CUs
+    bool promote = true;               // hybrid: stragglers move to the quad form early
     TailSet ts;
     std::vector<double> mratio;        // per unit (host copy of d_mratio)
     int group_off[kNumAllCfg + 1] = {0};
@@ -576,7 +577,7 @@ inline int tail_q(int n) { return n <= kQuadNT[3] / 4 ? 3 : 4; }
 // the units still running when the split happens (an eighth of the CUs)
 inline int tail_split_units(int n_cu) { return std::max(1, n_cu / 8); }
 static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::vector<double> &rem, int n_cu,
-                   std::vector<char> &split) {
+                   std::vector<char> &split, const std::vector<char> *only = nullptr) {
     TailSet &t = e->ts;
     t.n = 0;
     t.total = 0;
@@ -584,7 +585,7 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
     split.assign(e->n_units, 0);
     std::vector<int> run;
     for (int u : e->lpt)
-        if (ctl[u].status == ST_RUNNING) run.push_back(u);
+        if (ctl[u].status == ST_RUNNING && (only == nullptr || (*only)[u])) run.push_back(u);
     int extra = n_cu - (int)run.size();
     if (run.empty() || extra <= 0) return 0;
     auto rows = [](int n) { return (n + 63) / 64; };
@@ -790,6 +791,23 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     std::vector<double> t_run0, rem;          // per unit: model time and attempts at the start of the run
     std::vector<long long> att_run0;
     ord.reserve(n_plain_units);
+    // Stragglers (hybrid, before the tail): the run proceeds in rounds of a
+    // growing length (0.25 s, then as long as the run so far, at most 1 s);
+    // after each, a unit whose projected time to t_end (its model time left
+    // over its progress rate in the round) exceeds 4 times the median is
+    // promoted for the rest of the run: it runs in the quad form, split by
+    // the same rows rule as the tail, in a tail set of at most a quarter of
+    // the CUs (a sixteenth in units, the rest for their children), launched
+    // ahead of the one-lane units.  Its trajectory does not change (the same
+    // bits in either form).
+    const bool promoting = splitting && e->promote;
+    const int promote_cus = std::max(1, n_cu / 4);
+    double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
+    if (const char *s = getenv("ABIE_HIP_PROMOTE_FACTOR")) promote_factor = atof(s);
+    if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
+    std::vector<char> promoted(e->n_units, 0);
+    std::vector<double> t_prev(e->n_units, 0.0);
+    double el_prev = 0.0, round_s = round0;
     for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
         if (round == 0) {
@@ -813,6 +831,36 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (max_seconds > 0 && el >= max_seconds) break;
         split.assign(e->n_units, 0);
+        if (promoting && !e->tail && round > 0 && el > el_prev) {
+            const double w = el - el_prev;
+            std::vector<double> pu(e->n_units, -1.0), all;
+            for (int u = 0; u < e->n_units; ++u) {
+                if (ctl[u].status != ST_RUNNING || e->desc[u].link_n > 1) continue;
+                const double dt = ctl[u].t - t_prev[u];
+                pu[u] = dt > 0 ? (ctl[u].t_end - ctl[u].t) * w / dt : 1e300;
+                all.push_back(pu[u]);
+            }
+            if (!all.empty()) {
+                std::nth_element(all.begin(), all.begin() + all.size() / 2, all.end());
+                const double med = all[all.size() / 2];
+                int live = 0;
+                std::vector<std::pair<double, int>> cand;
+                for (int u = 0; u < e->n_units; ++u) {
+                    if (pu[u] < 0) continue;
+                    if (promoted[u]) ++live;
+                    else if (e->alt_quad[u] >= 0 && pu[u] > promote_factor * med) cand.push_back({pu[u], u});
+                }
+                std::sort(cand.rbegin(), cand.rend());
+                // a quarter of the budget in units, the rest for their children
+                for (auto &cu : cand) {
+                    if (live >= std::max(1, promote_cus / 4)) break;
+                    promoted[cu.second] = 1;
+                    ++live;
+                }
+            }
+        }
+        for (int u = 0; u < e->n_units; ++u) t_prev[u] = ctl[u].t;
+        el_prev = el;
         if (hybrid && !e->tail && n_run_units <= n_cu) {
             e->tail = true;
             for (int u = 0; u < e->n_units; ++u)
@@ -841,6 +889,20 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
                 if (rc) return rc;
             }
         }
+        bool any_promoted = false;
+        if (promoting && !e->tail)
+            for (int u = 0; u < e->n_units && !any_promoted; ++u)
+                any_promoted = promoted[u] && ctl[u].status == ST_RUNNING;
+        if (any_promoted) {
+            rem.assign(e->n_units, 0.0);
+            for (int u = 0; u < e->n_units; ++u) {
+                const double span = ctl[u].t_end - t_run0[u], done_t = ctl[u].t - t_run0[u];
+                const double att = (double)(ctl[u].n_attempts - att_run0[u]);
+                rem[u] = (span > 0 && done_t > 0 && att > 0) ? att * std::max(0.0, span - done_t) / done_t : 1e30;
+            }
+            int rc = ts_plan(e, ctl, rem, promote_cus, split, &promoted);
+            if (rc) return rc;
+        }
         int goff[kNumAllCfg + 1];
         ord.clear();
         for (int c = 0; c < kNumAllCfg; ++c) {
@@ -861,10 +923,24 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         a.tail_units = !hybrid ? 0
                        : !e->tail ? n_cu
                        : (splitting && !e->tail_split_done) ? tail_split_units(n_cu) : 0;
-        if (max_seconds > 0) {
+        const bool rounds = promoting && !e->tail;
+        if (max_seconds > 0 || rounds) {
             long long now = 0;
             HIP_TRY(read_device_clock(&now, e->d_clock.p, e->streams[0]));
-            a.deadline_tick = now + (long long)((max_seconds - el) * 1e3 * (double)e->clock_khz);
+            double left = max_seconds > 0 ? max_seconds - el : 1e30;
+            if (rounds) {
+                left = std::min(left, round_s);
+                round_s = std::min(1.0, std::max(round0, el + round_s));
+            }
+            a.deadline_tick = now + (long long)(left * 1e3 * (double)e->clock_khz);
+        }
+        // the tail set first (promoted stragglers get their CUs before the
+        // one-lane units fill the rest)
+        if (e->ts.n > 0) {
+            std::vector<int> all(e->ts.n);
+            std::iota(all.begin(), all.end(), 0);
+            int rc = ts_launch(e, tail_args(e, a), all);
+            if (rc) return rc;
         }
         for (int c = 0; c < kNumAllCfg; ++c) {
             const int nb = goff[c + 1] - goff[c];
@@ -873,12 +949,6 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             ac.order = e->d_order.p + goff[c];
             if (cfg_is_quad(c)) HIP_TRY(launch_gr15q(c - kNumCfg, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
             else HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
-        }
-        if (e->ts.n > 0) {
-            std::vector<int> all(e->ts.n);
-            std::iota(all.begin(), all.end(), 0);
-            int rc = ts_launch(e, tail_args(e, a), all);
-            if (rc) return rc;
         }
         for (int c = 0; c < kNumAllCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
         if (e->ts.n > 0) {
@@ -931,6 +1001,7 @@ int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, con
     const char *ev_env = getenv("ABIE_HIP_EVENTS");
     if (const char *q = getenv("ABIE_HIP_QUAD")) e->quad = std::min(2, std::max(0, atoi(q)));
     if (const char *q = getenv("ABIE_HIP_TAIL_SPLIT")) e->tail_split = atoi(q) != 0;
+    if (const char *q = getenv("ABIE_HIP_PROMOTE")) e->promote = atoi(q) != 0;
     if (ev_env && atoi(ev_env) != 0) e->quad = 0;
     int rc = ens_build(e, offsets, pos, vel, m, r, nullptr);
     if (rc == 0) rc = ens_rings(e, 0, 0);

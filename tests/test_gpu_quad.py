@@ -162,3 +162,30 @@ def test_tail_split_same_bits(monkeypatch, force_mode, controller):
         assert r["done"] and np.array_equal(r["t"], ref["t"]) and np.array_equal(r["st"], ref["st"])
         assert same_bits(r["p"], ref["p"]) and same_bits(r["v"], ref["v"]), chunk
         assert np.array_equal(r["cnt"], ref["cnt"]), chunk
+
+
+@pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "learned")])
+def test_promoted_stragglers_same_bits(monkeypatch, force_mode, controller):
+    """Straggler promotion during the one-lane phase (more units than CUs):
+    the run proceeds in rounds, and units projected to finish late move to
+    the quad form in a tail set of their own (split by the rows rule) for the
+    rest of the run.  Forced here with 5-ms rounds and every unit above the
+    median promoted (up to a quarter of the CUs): identical bits, times,
+    statuses and counters with promotion off."""
+    import os
+    from abie_amd import ics
+    w = None
+    if controller == "learned":
+        w = np.load(os.path.join(os.path.dirname(__file__), "..", "abie_amd", "data", "proposer.npz"))["weights"]
+    rng = np.random.default_rng(5)
+    systems = [ics.planetary_system(5200 + k, int(rng.integers(3, 40)), int(rng.integers(0, 300))) for k in range(300)]
+    monkeypatch.setenv("ABIE_HIP_PROMOTE", "0")
+    ref = _run(systems, 0.3, force_mode, controller, proposer=w)
+    monkeypatch.setenv("ABIE_HIP_PROMOTE", "1")
+    monkeypatch.setenv("ABIE_HIP_PROMOTE_FACTOR", "1.0")
+    monkeypatch.setenv("ABIE_HIP_ROUND_SECONDS", "0.005")
+    r = _run(systems, 0.3, force_mode, controller, proposer=w)
+    assert ref["done"] and r["done"]
+    assert np.array_equal(r["t"], ref["t"]) and np.array_equal(r["st"], ref["st"])
+    assert same_bits(r["p"], ref["p"]) and same_bits(r["v"], ref["v"])
+    assert np.array_equal(r["cnt"], ref["cnt"])
