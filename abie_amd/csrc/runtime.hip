@@ -9,6 +9,7 @@
 #include <array>
 #include <chrono>
 #include <numeric>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -60,6 +61,13 @@ struct DBuf {
         n = 0;
     }
 };
+
+// index of the one-lane (256 threads, one body each) configuration
+static int one_lane_256() {
+    for (int c = 0; c < kNumCfg; ++c)
+        if (kCfgs[c].nt == 256 && kCfgs[c].bpt == 1) return c;
+    return 0;
+}
 
 static int device_cu_count(int dev) {
     int v = 0;
@@ -142,7 +150,6 @@ struct abie_ensemble {
     // the quad form from the start.  Off when the event scan is asked for.
     int quad = 1;
     bool tail = false;                 // hybrid: the quad form has taken over
-    bool tail_split_done = false;      // hybrid: the last units run split
     std::vector<int> cur_cfg;          // per unit: the form of its next launch
     std::vector<int> alt_quad;         // per unit: its quad group size, or -1
     std::vector<int> lpt;              // units that are not linked slices, longest first
@@ -150,10 +157,16 @@ struct abie_ensemble {
     DBuf<long long> d_clock;
     bool tail_split = true;            // hybrid: split the tail's units over idle CUs
     bool promote = true;               // hybrid: stragglers move to the quad form early
+    bool trace_sched = false;          // ABIE_HIP_TRACE_SCHED: one stderr line per launch round
     TailSet ts;
     std::vector<double> mratio;        // per unit (host copy of d_mratio)
     int group_off[kNumAllCfg + 1] = {0};
-    hipStream_t streams[kNumAllCfg] = {};
+    // Launch lanes: four streams created one after the other, which gives
+    // them distinct hardware queues (HIP's default is four per process).
+    // Launches on streams that share a queue run one after the other, so a
+    // round's launches are assigned to the lanes, heaviest first.
+    static constexpr int kLanes = 4;
+    hipStream_t streams[kLanes] = {};
     DBuf<SysDesc> d_sys;
     DBuf<int> d_order, d_orig, d_inv;
     DBuf<SysCtl> d_ctl;
@@ -264,9 +277,14 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     auto quad_of = [&](int n) {
         return (e->ctrl != ABIE_CTRL_PLAIN && e->quad != 0 && n <= kQuadMaxBodies) ? pick_quad(n) : -1;
     };
+    // the hybrid schedule runs every one-lane unit of the massive-only
+    // controllers (at most 256 bodies) in the (256, 1) shape: one launch,
+    // so that it never waits on a hardware queue behind another shape's
     auto unit_cfg = [&](int n, int nm) {
         const int q = quad_of(n);
-        return (q >= 0 && e->quad == 2) ? kNumCfg + q : pick_cfg(n, nm);
+        if (q >= 0 && e->quad == 2) return kNumCfg + q;
+        if (q >= 0 && e->quad == 1 && cfg_fits(one_lane_256(), n, nm)) return one_lane_256();
+        return pick_cfg(n, nm);
     };
     const int U = (int)units.size();
     e->n_units = U;
@@ -435,13 +453,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         c.status = ST_RUNNING;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
-    // the tail's two streams (quad groups of 512 and 1,024 threads) first:
-    // streams created one after the other get distinct hardware queues
-    const int first[2] = {kNumCfg + 3, kNumCfg + 4};
-    for (int c : first)
-        if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
-    for (int c = 0; c < kNumAllCfg; ++c)
-        if (!e->streams[c]) HIP_TRY(hipStreamCreateWithFlags(&e->streams[c], hipStreamNonBlocking));
+    for (auto &st : e->streams)
+        if (!st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     return 0;
 }
 
@@ -553,20 +566,21 @@ static GR15Args tail_args(abie_ensemble *e, const GR15Args &base) {
     return a;
 }
 
-// Split the running units of the quad form over at most n_cu units in all
-// (one per CU, so that each child runs on a CU of its own).  Each child
+// Move the running units of the quad form (those in *only, if given) to the
+// tail set, split over at most n_cu units in all (one per CU).  Each child
 // carries all of its parent's massive bodies and a share of its test
-// particles (at least kTailMinTP).  A unit's evaluation latency steps with
-// the wave rows each SIMD runs, ceil(n / 64) for n bodies at four lanes per
-// body, so a split only pays where its children need fewer rows than the
-// parent: the unit with the most remaining time -- attempts still to come
-// (rem[u], from its progress in this run) times its rows times (16 + its
-// massive bodies) -- repeatedly moves to the next lower row count while the
-// idle CUs allow it.  Every unit that can run in the quad form moves to the
-// tail's arrays, as children or whole, so that the rest of the run is one
-// launch per tail group size; split[u] = 1 for those units.  Units whose
-// norms cover every body (norm_all) move whole: their step depends on every
-// body.
+// particles (at least kTailMinTP), and makes exactly the parent's steps (test
+// particles never enter the massive-only norms).  A unit's evaluation
+// latency steps with the wave rows each SIMD runs, ceil(n / 64) for n bodies
+// at four lanes per body, so a split only pays where its children need fewer
+// rows than the parent: the unit with the most remaining time -- attempts
+// still to come (rem[u]) times its rows times (16 + its massive bodies) --
+// repeatedly moves to the next lower row count while the CUs allow it.
+// max_rows > 0: every unit of the set is first split to at most that many
+// rows (one group size), most remaining time first; a unit the CUs left
+// cannot split that far stays out.  Copies the children's bodies out of the
+// ensemble's arrays; split[u] = 1 for the units moved.  Units whose norms
+// cover every body (norm_all) move whole: their step depends on every body.
 constexpr int kTailMinTP = 4;
 // The quad group a unit of n bodies runs in at the tail: 512 threads up to
 // 128 bodies, 1,024 beyond (waves without bodies end at the kernel's start).
@@ -574,10 +588,10 @@ constexpr int kTailMinTP = 4;
 // on the two streams created first: launches on streams that share a
 // hardware queue run one after the other.
 inline int tail_q(int n) { return n <= kQuadNT[3] / 4 ? 3 : 4; }
-// the units still running when the split happens (an eighth of the CUs)
-inline int tail_split_units(int n_cu) { return std::max(1, n_cu / 8); }
+// longest round at the tail (a round also ends when a quarter of its units finish)
+constexpr double kTailRoundSeconds = 0.2;
 static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::vector<double> &rem, int n_cu,
-                   std::vector<char> &split, const std::vector<char> *only = nullptr) {
+                   std::vector<char> &split, const std::vector<char> *only = nullptr, int max_rows = 0) {
     TailSet &t = e->ts;
     t.n = 0;
     t.total = 0;
@@ -585,24 +599,50 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
     split.assign(e->n_units, 0);
     std::vector<int> run;
     for (int u : e->lpt)
-        if (ctl[u].status == ST_RUNNING && (only == nullptr || (*only)[u])) run.push_back(u);
-    int extra = n_cu - (int)run.size();
-    if (run.empty() || extra <= 0) return 0;
+        if (ctl[u].status == ST_RUNNING && e->alt_quad[u] >= 0 && (only == nullptr || (*only)[u]))
+            run.push_back(u);
     auto rows = [](int n) { return (n + 63) / 64; };
     std::vector<int> kk(e->n_units, 1);
+    int extra = n_cu - (int)run.size();
+    if (max_rows > 0) {
+        // every unit of the set runs in at most max_rows wave rows (a single
+        // group size): most remaining time first, a unit the CUs left
+        // cannot split that far stays out of the set
+        std::stable_sort(run.begin(), run.end(), [&](int a, int b) { return rem[a] > rem[b]; });
+        std::vector<int> keep;
+        int used = 0;
+        for (int u : run) {
+            const SysDesc &d = e->desc[u];
+            int want = 1;
+            if (rows(d.n) > max_rows) {
+                const int nt = d.n - d.nm, room = 64 * max_rows - d.nm;
+                want = (!d.norm_all && room >= kTailMinTP && nt > 0) ? (nt + room - 1) / room : 0;
+            }
+            if (want == 0 || used + want > n_cu) continue;
+            kk[u] = want;
+            used += want;
+            keep.push_back(u);
+        }
+        run.swap(keep);
+        extra = n_cu - used;
+    }
+    if (run.empty()) return 0;
+    auto child_n = [&](int u) {
+        const SysDesc &d = e->desc[u];
+        const int nt = d.n - d.nm;
+        return d.nm + (nt > 0 ? (nt + kk[u] - 1) / kk[u] : 0);
+    };
     std::vector<int> cand;
     for (int u : run) {
         const SysDesc &d = e->desc[u];
-        if (e->alt_quad[u] >= 0 && !d.norm_all && d.link_n <= 1 && d.n - d.nm >= 2 * kTailMinTP && rows(d.n) > 1)
-            cand.push_back(u);
+        if (!d.norm_all && d.link_n <= 1 && d.n - d.nm >= 2 * kTailMinTP && rows(child_n(u)) > 1) cand.push_back(u);
     }
     while (extra > 0 && !cand.empty()) {
         int bi = 0;
         double bt = -1.0;
         for (int i = 0; i < (int)cand.size(); ++i) {
             const SysDesc &d = e->desc[cand[i]];
-            const int nt = d.n - d.nm, chunk = (nt + kk[cand[i]] - 1) / kk[cand[i]];
-            const double tt = rem[cand[i]] * rows(d.nm + chunk) * (16.0 + d.nm);
+            const double tt = rem[cand[i]] * rows(child_n(cand[i])) * (16.0 + d.nm);
             if (tt > bt) {
                 bt = tt;
                 bi = i;
@@ -611,7 +651,7 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
         const int u = cand[bi];
         const SysDesc &d = e->desc[u];
         const int nt = d.n - d.nm;
-        const int r = rows(d.nm + (nt + kk[u] - 1) / kk[u]) - 1;  // the next lower row count
+        const int r = rows(child_n(u)) - 1;  // the next lower row count
         const int room = 64 * r - d.nm;                              // test particles per child there
         const int want = room >= kTailMinTP ? (nt + room - 1) / room : 0;
         if (r < 1 || want == 0 || want - kk[u] > extra) {
@@ -626,7 +666,6 @@ static int ts_plan(abie_ensemble *e, const std::vector<SysCtl> &ctl, const std::
     std::vector<SysCtl> cctl;
     for (int u : run) {
         const SysDesc &d = e->desc[u];
-        if (e->alt_quad[u] < 0) continue;
         const int nt = d.n - d.nm;
         const int chunk = nt > 0 ? (nt + kk[u] - 1) / kk[u] : 0;
         const int k = nt > 0 ? (nt + chunk - 1) / chunk : 1;
@@ -699,7 +738,7 @@ static int ts_launch(abie_ensemble *e, const GR15Args &a, const std::vector<int>
         if (nb == 0) continue;
         GR15Args ac = a;
         ac.order = t.order.p + goff[q];
-        HIP_TRY(launch_gr15q(q, e->force_mode, e->ctrl, ac, nb, e->streams[kNumCfg + q]));
+        HIP_TRY(launch_gr15q(q, e->force_mode, e->ctrl, ac, nb, e->streams[q == kNumQuad - 1 ? 1 : 0]));
     }
     return 0;
 }
@@ -734,7 +773,7 @@ static int ts_finish(abie_ensemble *e, const GR15Args &base) {
         a.running = nullptr;
         int rc = ts_launch(e, a, lag);
         if (rc) return rc;
-        for (int q = 0; q < kNumQuad; ++q) HIP_TRY(hipStreamSynchronize(e->streams[kNumCfg + q]));
+        for (auto st : e->streams) HIP_TRY(hipStreamSynchronize(st));
     }
     HIP_TRY(launch_body_scatter(t.d_smap.p, t.total, tail_bodies(t), main_bodies(e), e->streams[0]));
     HIP_TRY(hipStreamSynchronize(e->streams[0]));
@@ -783,7 +822,6 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     const int n_plain_units = e->group_off[kNumAllCfg];  // units that are not linked slices
     if (hybrid) {  // every run starts in the one-lane form (the same bits either way)
         e->tail = false;
-        e->tail_split_done = false;
         for (int u = 0; u < e->n_units; ++u) e->cur_cfg[u] = e->desc[u].cfg;
     }
     std::vector<int> ord;
@@ -866,41 +904,41 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             for (int u = 0; u < e->n_units; ++u)
                 if (e->alt_quad[u] >= 0) e->cur_cfg[u] = kNumCfg + tail_q(e->desc[u].n);
         }
-        if (splitting && e->tail && !e->tail_split_done && n_run_units <= tail_split_units(n_cu)) {
-            e->tail_split_done = true;
-            {
-                // remaining attempts of each unit from its progress so far in
-                // this run (a unit that has not advanced yet counts as the
-                // one with the most progress to make)
-                rem.assign(e->n_units, 0.0);
-                double worst = 1.0;
-                for (int u = 0; u < e->n_units; ++u) {
-                    const double span = ctl[u].t_end - t_run0[u], done_t = ctl[u].t - t_run0[u];
-                    const double att = (double)(ctl[u].n_attempts - att_run0[u]);
-                    if (ctl[u].status == ST_RUNNING && span > 0 && done_t > 0 && att > 0) {
-                        rem[u] = att * std::max(0.0, span - done_t) / done_t;
-                        worst = std::max(worst, rem[u]);
-                    } else {
-                        rem[u] = -1.0;
-                    }
+        // remaining attempts of each unit from its progress so far in this
+        // run (a unit that has not advanced yet counts as the one with the
+        // most progress to make)
+        auto remaining = [&]() {
+            rem.assign(e->n_units, 0.0);
+            double worst = 1.0;
+            for (int u = 0; u < e->n_units; ++u) {
+                const double span = ctl[u].t_end - t_run0[u], done_t = ctl[u].t - t_run0[u];
+                const double att = (double)(ctl[u].n_attempts - att_run0[u]);
+                if (ctl[u].status == ST_RUNNING && span > 0 && done_t > 0 && att > 0) {
+                    rem[u] = att * std::max(0.0, span - done_t) / done_t;
+                    worst = std::max(worst, rem[u]);
+                } else {
+                    rem[u] = -1.0;
                 }
-                for (auto &r : rem) if (r < 0) r = worst;
-                int rc = ts_plan(e, ctl, rem, n_cu, split);
-                if (rc) return rc;
             }
-        }
+            for (auto &r : rem)
+                if (r < 0) r = worst;
+        };
+        // at the tail every round moves the running units to the tail set,
+        // split by the rows rule over the CUs they leave idle (re-planned as
+        // units finish); before it, the promoted units alone, in at most two
+        // wave rows (one group size, one launch beside the one-lane one)
+        const bool tail_rounds = splitting && e->tail;
         bool any_promoted = false;
         if (promoting && !e->tail)
             for (int u = 0; u < e->n_units && !any_promoted; ++u)
                 any_promoted = promoted[u] && ctl[u].status == ST_RUNNING;
-        if (any_promoted) {
-            rem.assign(e->n_units, 0.0);
-            for (int u = 0; u < e->n_units; ++u) {
-                const double span = ctl[u].t_end - t_run0[u], done_t = ctl[u].t - t_run0[u];
-                const double att = (double)(ctl[u].n_attempts - att_run0[u]);
-                rem[u] = (span > 0 && done_t > 0 && att > 0) ? att * std::max(0.0, span - done_t) / done_t : 1e30;
-            }
-            int rc = ts_plan(e, ctl, rem, promote_cus, split, &promoted);
+        if (tail_rounds) {
+            remaining();
+            int rc = ts_plan(e, ctl, rem, n_cu, split);
+            if (rc) return rc;
+        } else if (any_promoted) {
+            remaining();
+            int rc = ts_plan(e, ctl, rem, promote_cus, split, &promoted, 2);
             if (rc) return rc;
         }
         int goff[kNumAllCfg + 1];
@@ -918,39 +956,67 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         GR15Args a = ens_args(e);
         a.budget_ticks = (long long)(group_seconds * 1e3 * (double)e->clock_khz);
         a.running = e->d_running.p;
-        // one-lane units yield to the quad form at the tail, quad units to
-        // the split once few remain
-        a.tail_units = !hybrid ? 0
-                       : !e->tail ? n_cu
-                       : (splitting && !e->tail_split_done) ? tail_split_units(n_cu) : 0;
-        const bool rounds = promoting && !e->tail;
+        // one-lane units yield to the quad form at the tail; at the tail a
+        // round ends once a quarter of its units have finished (their CUs go
+        // to the others' children in the next round's plan)
+        a.tail_units = !hybrid ? 0 : !e->tail ? n_cu : tail_rounds ? 3 * n_launch / 4 : 0;
+        const bool rounds = (promoting && !e->tail) || tail_rounds;
         if (max_seconds > 0 || rounds) {
             long long now = 0;
             HIP_TRY(read_device_clock(&now, e->d_clock.p, e->streams[0]));
             double left = max_seconds > 0 ? max_seconds - el : 1e30;
-            if (rounds) {
+            if (tail_rounds) {
+                left = std::min(left, kTailRoundSeconds);
+            } else if (rounds) {
                 left = std::min(left, round_s);
                 round_s = std::min(1.0, std::max(round0, el + round_s));
             }
             a.deadline_tick = now + (long long)(left * 1e3 * (double)e->clock_khz);
         }
-        // the tail set first (promoted stragglers get their CUs before the
-        // one-lane units fill the rest)
+        if (e->trace_sched) {
+            int np = 0;
+            for (int u = 0; u < e->n_units; ++u) np += promoted[u] && ctl[u].status == ST_RUNNING;
+            fprintf(stderr, "[abie sched] round %d t %.3f s: %d units running, %d promoted, %d launched in "
+                    "the arrays, %d in the tail set (%lld bodies), tail %d\n", round, el, n_run_units, np,
+                    (int)ord.size(), e->ts.n, e->ts.total, (int)e->tail);
+        }
+        // the tail set first: promoted stragglers take their CUs while the
+        // GPU is idle (a quad group of up to 512 threads needs a CU's whole
+        // register file, which a CU running one-lane groups never frees while
+        // more of them wait), and only then do the one-lane units fill the rest
         if (e->ts.n > 0) {
             std::vector<int> all(e->ts.n);
             std::iota(all.begin(), all.end(), 0);
             int rc = ts_launch(e, tail_args(e, a), all);
             if (rc) return rc;
+            if (ord.size() > 0) std::this_thread::sleep_for(std::chrono::microseconds(300));
         }
-        for (int c = 0; c < kNumAllCfg; ++c) {
-            const int nb = goff[c + 1] - goff[c];
-            if (nb == 0) continue;
-            GR15Args ac = a;
-            ac.order = e->d_order.p + goff[c];
-            if (cfg_is_quad(c)) HIP_TRY(launch_gr15q(c - kNumCfg, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
-            else HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[c]));
+        // the groups' launches over the lanes, heaviest first (work per
+        // evaluation, sum of n * N_m), each to the least loaded lane; the
+        // tail set's launches hold lanes 0 and 1
+        {
+            std::vector<std::pair<double, int>> grp;
+            for (int c = 0; c < kNumAllCfg; ++c) {
+                double w = 0.0;
+                for (int i = goff[c]; i < goff[c + 1]; ++i) w += (double)e->desc[ord[i]].n * std::max(1, e->desc[ord[i]].nm);
+                if (goff[c + 1] > goff[c]) grp.push_back({w, c});
+            }
+            std::sort(grp.rbegin(), grp.rend());
+            double load[abie_ensemble::kLanes] = {0.0, 0.0, 0.0, 0.0};
+            if (e->ts.n > 0) load[0] = load[1] = 1e300;
+            for (auto &g : grp) {
+                const int c = g.second, nb = goff[c + 1] - goff[c];
+                int l = 0;
+                for (int k = 1; k < abie_ensemble::kLanes; ++k)
+                    if (load[k] < load[l]) l = k;
+                load[l] += g.first;
+                GR15Args ac = a;
+                ac.order = e->d_order.p + goff[c];
+                if (cfg_is_quad(c)) HIP_TRY(launch_gr15q(c - kNumCfg, e->force_mode, e->ctrl, ac, nb, e->streams[l]));
+                else HIP_TRY(launch_gr15(c, e->force_mode, e->ctrl, ac, nb, e->streams[l]));
+            }
         }
-        for (int c = 0; c < kNumAllCfg; ++c) HIP_TRY(hipStreamSynchronize(e->streams[c]));
+        for (auto st : e->streams) HIP_TRY(hipStreamSynchronize(st));
         if (e->ts.n > 0) {
             int rc = ts_finish(e, a);
             if (rc) return rc;
@@ -961,11 +1027,11 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             if (ctl[e->order[lk[1]]].status != ST_RUNNING) continue;
             GR15Args ac = a;
             ac.order = e->d_order.p + lk[1];
-            const hipError_t err = launch_gr15(lk[0], e->force_mode, e->ctrl, ac, lk[2], e->streams[lk[0]], true);
+            const hipError_t err = launch_gr15(lk[0], e->force_mode, e->ctrl, ac, lk[2], e->streams[0], true);
             if (err != hipSuccess)
                 return fail(ABIE_HIP_EHIP, "cooperative launch of %d linked slices failed: %s (the slices of one "
                             "system must be resident together)", lk[2], hipGetErrorString(err));
-            HIP_TRY(hipStreamSynchronize(e->streams[lk[0]]));
+            HIP_TRY(hipStreamSynchronize(e->streams[0]));
         }
     }
     if (done_out) *done_out = done;
@@ -1002,6 +1068,7 @@ int abie_ensemble_create(abie_ensemble **out, int n_sys, const int *offsets, con
     if (const char *q = getenv("ABIE_HIP_QUAD")) e->quad = std::min(2, std::max(0, atoi(q)));
     if (const char *q = getenv("ABIE_HIP_TAIL_SPLIT")) e->tail_split = atoi(q) != 0;
     if (const char *q = getenv("ABIE_HIP_PROMOTE")) e->promote = atoi(q) != 0;
+    if (const char *q = getenv("ABIE_HIP_TRACE_SCHED")) e->trace_sched = atoi(q) != 0;
     if (ev_env && atoi(ev_env) != 0) e->quad = 0;
     int rc = ens_build(e, offsets, pos, vel, m, r, nullptr);
     if (rc == 0) rc = ens_rings(e, 0, 0);
