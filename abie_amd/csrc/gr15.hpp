@@ -49,6 +49,9 @@ struct SysCtl {
     // a unit stops (state kept) once its attempt count reaches this, 0: no
     // such stop (the host brings the slices of a split unit to one attempt)
     long long stop_attempt;
+    // device wall-clock ticks the unit has spent in launches, summed (its
+    // progress rate for the host's straggler promotion)
+    long long busy_ticks;
 };
 
 enum : int { ST_RUNNING = 0, ST_CE = 1, ST_COLL = 2, ST_DONE = 3, ST_ERROR = -1 };

@@ -1390,6 +1390,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) 
         ctl->n_prop_used += n_pu;
         ctl->n_prop_rej += n_pr;
         if (ctl->start_tick == 0) ctl->start_tick = t_enter;
+        ctl->busy_ticks += wall_clock64() - t_enter;
         if (status != ST_RUNNING) {
             ctl->done_tick = wall_clock64();
             if (A.running != nullptr) atomicSub(A.running, 1);

@@ -832,12 +832,12 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     // Stragglers (hybrid, before the tail): the run proceeds in rounds of a
     // growing length (0.25 s, then as long as the run so far, at most 1 s);
     // after each, a unit whose projected time to t_end (its model time left
-    // over its progress rate in the round) exceeds 4 times the median is
-    // promoted for the rest of the run: it runs in the quad form, split by
-    // the same rows rule as the tail, in a tail set of at most a quarter of
-    // the CUs (a sixteenth in units, the rest for their children), launched
-    // ahead of the one-lane units.  Its trajectory does not change (the same
-    // bits in either form).
+    // over its progress rate while it ran in the round) exceeds 4 times the
+    // median is promoted for the rest of the run, most remaining time first:
+    // it runs in the quad form, in a tail set of at most a quarter of the CUs
+    // (one per unit, the rest splitting the units with the most remaining
+    // time by the rows rule), launched ahead of the one-lane units.  Its
+    // trajectory does not change (the same bits in either form).
     const bool promoting = splitting && e->promote;
     const int promote_cus = std::max(1, n_cu / 4);
     double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
@@ -845,6 +845,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
     std::vector<char> promoted(e->n_units, 0);
     std::vector<double> t_prev(e->n_units, 0.0);
+    std::vector<long long> busy_prev(e->n_units, 0);
     double el_prev = 0.0, round_s = round0;
     for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
@@ -870,12 +871,20 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         if (max_seconds > 0 && el >= max_seconds) break;
         split.assign(e->n_units, 0);
         if (promoting && !e->tail && round > 0 && el > el_prev) {
+            // only units that ran in the last round have a rate (with more
+            // units than the GPU holds at once, many have not started yet)
             const double w = el - el_prev;
             std::vector<double> pu(e->n_units, -1.0), all;
+            // projected run time left: model time left over the rate per
+            // tick the unit was running in the last round (units that ran for
+            // less than a tenth of the round are not judged)
+            const double min_ticks = 0.1 * w * 1e3 * (double)e->clock_khz;
             for (int u = 0; u < e->n_units; ++u) {
                 if (ctl[u].status != ST_RUNNING || e->desc[u].link_n > 1) continue;
+                const double busy = (double)(ctl[u].busy_ticks - busy_prev[u]);
+                if (busy < min_ticks) continue;
                 const double dt = ctl[u].t - t_prev[u];
-                pu[u] = dt > 0 ? (ctl[u].t_end - ctl[u].t) * w / dt : 1e300;
+                pu[u] = dt > 0 ? (ctl[u].t_end - ctl[u].t) * busy / dt : 1e300;
                 all.push_back(pu[u]);
             }
             if (!all.empty()) {
@@ -889,15 +898,19 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
                     else if (e->alt_quad[u] >= 0 && pu[u] > promote_factor * med) cand.push_back({pu[u], u});
                 }
                 std::sort(cand.rbegin(), cand.rend());
-                // a quarter of the budget in units, the rest for their children
+                // one CU per promoted unit at least; the CUs left over split
+                // the ones with the most remaining time (ts_plan)
                 for (auto &cu : cand) {
-                    if (live >= std::max(1, promote_cus / 4)) break;
+                    if (live >= promote_cus) break;
                     promoted[cu.second] = 1;
                     ++live;
                 }
             }
         }
-        for (int u = 0; u < e->n_units; ++u) t_prev[u] = ctl[u].t;
+        for (int u = 0; u < e->n_units; ++u) {
+            t_prev[u] = ctl[u].t;
+            busy_prev[u] = ctl[u].busy_ticks;
+        }
         el_prev = el;
         if (hybrid && !e->tail && n_run_units <= n_cu) {
             e->tail = true;
@@ -925,8 +938,9 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         };
         // at the tail every round moves the running units to the tail set,
         // split by the rows rule over the CUs they leave idle (re-planned as
-        // units finish); before it, the promoted units alone, in at most two
-        // wave rows (one group size, one launch beside the one-lane one)
+        // units finish); before it, the promoted units alone, within their
+        // quarter of the CUs (two launches on lanes 0 and 1, beside the
+        // one-lane launch on lane 2)
         const bool tail_rounds = splitting && e->tail;
         bool any_promoted = false;
         if (promoting && !e->tail)
@@ -938,7 +952,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             if (rc) return rc;
         } else if (any_promoted) {
             remaining();
-            int rc = ts_plan(e, ctl, rem, promote_cus, split, &promoted, 2);
+            int rc = ts_plan(e, ctl, rem, promote_cus, split, &promoted);
             if (rc) return rc;
         }
         int goff[kNumAllCfg + 1];
@@ -979,6 +993,21 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             fprintf(stderr, "[abie sched] round %d t %.3f s: %d units running, %d promoted, %d launched in "
                     "the arrays, %d in the tail set (%lld bodies), tail %d\n", round, el, n_run_units, np,
                     (int)ord.size(), e->ts.n, e->ts.total, (int)e->tail);
+            if (const char *ts = getenv("ABIE_HIP_TRACE_SYS")) {
+                const int sy = atoi(ts);
+                if (sy >= 0 && sy < e->n_sys) {
+                    const int u = e->sys_unit0[sy];
+                    int kids = 0, kid_n = 0;
+                    for (int c = 0; c < e->ts.n; ++c)
+                        if (e->ts.parent[c] == u) {
+                            ++kids;
+                            kid_n = std::max(kid_n, e->ts.desc[c].n);
+                        }
+                    fprintf(stderr, "[abie sched]   system %d unit %d: t %.4f, %lld evals, %d bodies, cfg %d, %d "
+                            "children of up to %d bodies\n", sy, u, ctl[u].t, ctl[u].n_evals, e->desc[u].n,
+                            e->cur_cfg[u], kids, kid_n);
+                }
+            }
         }
         // the tail set first: promoted stragglers take their CUs while the
         // GPU is idle (a quad group of up to 512 threads needs a CU's whole
