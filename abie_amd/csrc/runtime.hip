@@ -839,7 +839,10 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     // time by the rows rule), launched ahead of the one-lane units.  Its
     // trajectory does not change (the same bits in either form).
     const bool promoting = splitting && e->promote;
-    const int promote_cus = std::max(1, n_cu / 4);
+    int promote_cus = std::max(1, n_cu / 4);
+    if (const char *s = getenv("ABIE_HIP_PROMOTE_CUS")) promote_cus = std::max(1, std::min(n_cu, atoi(s)));
+    int promote_units = promote_cus;
+    if (const char *s = getenv("ABIE_HIP_PROMOTE_UNITS")) promote_units = std::max(1, std::min(promote_cus, atoi(s)));
     double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
     if (const char *s = getenv("ABIE_HIP_PROMOTE_FACTOR")) promote_factor = atof(s);
     if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
@@ -898,10 +901,10 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
                     else if (e->alt_quad[u] >= 0 && pu[u] > promote_factor * med) cand.push_back({pu[u], u});
                 }
                 std::sort(cand.rbegin(), cand.rend());
-                // one CU per promoted unit at least; the CUs left over split
-                // the ones with the most remaining time (ts_plan)
+                // one CU per promoted unit; CUs left over split the ones
+                // with the most remaining time (ts_plan)
                 for (auto &cu : cand) {
-                    if (live >= promote_cus) break;
+                    if (live >= promote_units) break;
                     promoted[cu.second] = 1;
                     ++live;
                 }
