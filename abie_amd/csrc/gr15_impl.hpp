@@ -296,14 +296,18 @@ __device__ __forceinline__ void pair_sel(double px, double py, double pz, const 
 // (P0 + P1) + (P2 + P3).  Both forms therefore give the same bits, and a
 // unit may move from one to the other between launches.
 //
-// sum_stride4 adds P_c at position p to (ax, ay, az), four sources per
-// iteration in two register sets that alternate: the next two sources are
-// read from LDS while the current two are summed (LDS reads return in
-// order, so each wait covers only the older reads).  The order of the
-// terms, and so the bits, are those of a plain loop.  MW: the wave holds
-// massive bodies (pair_sel with self = (j == me), corr = massive).  The
-// reads ahead are clamped to the last source, so none leaves the list.
-template <int FM, bool MW>
+// sum_stride4 adds P_c at position p to (ax, ay, az), 2 P sources per
+// iteration in two register sets of P that alternate: the next P sources are
+// read from LDS while the current P are summed (LDS reads return in order,
+// so each wait covers only the older reads).  The order of the terms, and so
+// the bits, are those of a plain loop whatever P is.  P = 2 for the quad
+// form (a lone wave per SIMD: the pairs in flight hide FP64 latency; 4 was
+// measured 10 % slower), P = 1 for the one-lane form (three waves per SIMD
+// hide it, and the freed registers spill less: 1-2 % more ensemble
+// throughput).  MW: the wave holds massive bodies (pair_sel with self =
+// (j == me), corr = massive).  The reads ahead are clamped to the last
+// source, so none leaves the list.
+template <int FM, bool MW, int P = 2>
 __device__ __forceinline__ void sum_stride4(const double *S, int c, int nm, int me, bool massive, double px,
                                             double py, double pz, double &ax, double &ay, double &az, Coef2 kc) {
     const int cnt = nm > c ? (nm - c + 3) >> 2 : 0;
@@ -317,27 +321,30 @@ __device__ __forceinline__ void sum_stride4(const double *S, int c, int nm, int 
 #pragma unroll
         for (int u = 0; u < 4; ++u) q[u] = Sp[16 * kk + u];
     };
-    double a0[4], a1[4], b0[4], b1[4];
-    load(a0, 0);
-    load(a1, 1);
+    double A[P][4], B[P][4];
+#pragma unroll
+    for (int i = 0; i < P; ++i) load(A[i], i);
     int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-        load(b0, k + 2);
-        load(b1, k + 3);
-        term(a0, k);
-        term(a1, k + 1);
-        load(a0, k + 4);
-        load(a1, k + 5);
-        term(b0, k + 2);
-        term(b1, k + 3);
+    for (; k + 2 * P <= cnt; k += 2 * P) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) load(B[i], k + P + i);
+#pragma unroll
+        for (int i = 0; i < P; ++i) term(A[i], k + i);
+#pragma unroll
+        for (int i = 0; i < P; ++i) load(A[i], k + 2 * P + i);
+#pragma unroll
+        for (int i = 0; i < P; ++i) term(B[i], k + P + i);
     }
-    // the remaining 0..3 sources: a0, a1 hold k and k + 1
-    if (k < cnt) term(a0, k);
-    if (k + 1 < cnt) term(a1, k + 1);
-    if (k + 2 < cnt) {
-        load(b0, k + 2);
-        term(b0, k + 2);
-    }
+    // the remaining 0 .. 2P - 1 sources: A holds k .. k + P - 1
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        if (k + i < cnt) term(A[i], k + i);
+#pragma unroll
+    for (int i = 0; i < P - 1; ++i)
+        if (k + P + i < cnt) {
+            load(B[i], k + P + i);
+            term(B[i], k + P + i);
+        }
 }
 
 // -------------------------------------------------------------- reductions
@@ -587,8 +594,8 @@ struct Engine {
 #pragma unroll 1
             for (int c = 0; c < 4; ++c) {
                 double a[3] = {0.0, 0.0, 0.0};
-                if (wave_massive) sum_stride4<FM, true>(S, c, nm, me, mass, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
-                else sum_stride4<FM, false>(S, c, nm, me, false, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
+                if (wave_massive) sum_stride4<FM, true, 1>(S, c, nm, me, mass, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
+                else sum_stride4<FM, false, 1>(S, c, nm, me, false, p[0][0], p[0][1], p[0][2], a[0], a[1], a[2], kc);
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     s01[q] = c == 0 ? a[q] : (c == 1 ? s01[q] + a[q] : s01[q]);
