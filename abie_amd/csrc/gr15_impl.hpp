@@ -301,10 +301,11 @@ __device__ __forceinline__ void pair_sel(double px, double py, double pz, const 
 // read from LDS while the current P are summed (LDS reads return in order,
 // so each wait covers only the older reads).  The order of the terms, and so
 // the bits, are those of a plain loop whatever P is.  P = 2 for the quad
-// form (a lone wave per SIMD: the pairs in flight hide FP64 latency; 4 was
-// measured 10 % slower), P = 1 for the one-lane form (three waves per SIMD
-// hide it, and the freed registers spill less: 1-2 % more ensemble
-// throughput).  MW: the wave holds massive bodies (pair_sel with self =
+// form's groups of up to 512 threads (a lone wave per SIMD: the pairs in
+// flight hide FP64 latency; 4 was measured 10 % slower), P = 1 for the
+// one-lane form and the 1,024-thread quad group (three or four waves per
+// SIMD hide it, and the freed registers spill less: 1-2 % more ensemble
+// throughput, 5 % lower latency of 256 such units at once).  MW: the wave holds massive bodies (pair_sel with self =
 // (j == me), corr = massive).  The reads ahead are clamped to the last
 // source, so none leaves the list.
 template <int FM, bool MW, int P = 2>

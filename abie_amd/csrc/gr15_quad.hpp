@@ -134,8 +134,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
         // massive waves: FP64 with the self-pair removed for massive sinks,
         // test particles of the same wave as in the other branch (one loop,
         // no divergence)
-        if (wave_massive) sum_stride4<FM, true>(S, c, nm, bi, massive, px, py, pz, ax, ay, az, kc);
-        else sum_stride4<FM, false>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
+        if (wave_massive) sum_stride4<FM, true, (NT >= 1024 ? 1 : 2)>(S, c, nm, bi, massive, px, py, pz, ax, ay, az, kc);
+        else sum_stride4<FM, false, (NT >= 1024 ? 1 : 2)>(S, c, nm, bi, false, px, py, pz, ax, ay, az, kc);
         ax = quad_sum(ax);
         ay = quad_sum(ay);
         az = quad_sum(az);
