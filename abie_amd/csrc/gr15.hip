@@ -137,6 +137,42 @@ hipError_t launch_body_scatter(const long long *map, long long n, const BodyArra
     return hipGetLastError();
 }
 
+// Concurrency probe of n streams: one single-wave kernel per stream counts
+// itself in and waits until all n have arrived, or until wait_ticks of the
+// constant-rate clock have passed, when it flags *late.  Launches on
+// streams that share a hardware queue run one after the other, so the first
+// of two such probes times out.  Every wait ends.
+__global__ void stream_probe_kernel(unsigned *count, unsigned n, long long wait_ticks, unsigned *late) {
+    if (threadIdx.x != 0) return;
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n) {
+        if (wall_clock64() - t0 > wait_ticks) {
+            __hip_atomic_fetch_or(late, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+hipError_t streams_concurrent(const hipStream_t *st, int n, unsigned *dev_buf, long long wait_ticks, bool *ok) {
+    hipError_t e = hipMemset(dev_buf, 0, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        hipLaunchKernelGGL(stream_probe_kernel, dim3(1), dim3(64), 0, st[i], dev_buf, (unsigned)n, wait_ticks,
+                           dev_buf + 1);
+        e = hipGetLastError();
+    }
+    for (int i = 0; i < n; ++i) {
+        const hipError_t f = hipStreamSynchronize(st[i]);
+        if (e == hipSuccess) e = f;
+    }
+    unsigned late = 1;
+    if (e == hipSuccess) e = hipMemcpy(&late, dev_buf + 1, sizeof(unsigned), hipMemcpyDeviceToHost);
+    *ok = e == hipSuccess && late == 0;
+    return e;
+}
+
 hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st) {
     hipLaunchKernelGGL(device_clock_kernel, dim3(1), dim3(1), 0, st, dev_buf);
     hipError_t e = hipGetLastError();

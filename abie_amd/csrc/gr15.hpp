@@ -175,6 +175,10 @@ hipError_t launch_body_scatter(const long long *map, long long n, const BodyArra
 
 // The device's constant-rate wall clock (wall_clock64) into *host_tick.
 hipError_t read_device_clock(long long *host_tick, long long *dev_buf, hipStream_t st);
+// *ok: the n streams ran a probe kernel each at the same time (no two share a
+// hardware queue); dev_buf holds 2 unsigned ints of scratch; each probe waits
+// at most wait_ticks of the constant-rate clock.
+hipError_t streams_concurrent(const hipStream_t *st, int n, unsigned *dev_buf, long long wait_ticks, bool *ok);
 hipError_t launch_energy(const SysDesc *sys, int n_sys, const double *x, const double *y, const double *z,
                          const double *vx, const double *vy, const double *vz, const double *m, double G,
                          double *energy, hipStream_t st);

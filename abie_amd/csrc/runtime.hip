@@ -167,6 +167,8 @@ struct abie_ensemble {
     // round's launches are assigned to the lanes, heaviest first.
     static constexpr int kLanes = 4;
     hipStream_t streams[kLanes] = {};
+    int lanes_distinct = 0;            // lanes that passed the concurrency probe
+    DBuf<unsigned> d_probe;
     DBuf<SysDesc> d_sys;
     DBuf<int> d_order, d_orig, d_inv;
     DBuf<SysCtl> d_ctl;
@@ -453,6 +455,30 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         c.status = ST_RUNNING;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
+    if (!e->streams[0]) {
+        // four lanes on distinct hardware queues: each new stream joins the
+        // lanes if all of them then run a probe kernel at the same time;
+        // one that shares a queue with a lane is kept aside (holding its
+        // queue, so that the next stream goes elsewhere) and released after
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->dev));
+        HIP_TRY(e->d_probe.reserve(2));
+        std::vector<hipStream_t> aside;
+        int have = 0;
+        for (int attempt = 0; attempt < 16 && have < abie_ensemble::kLanes; ++attempt) {
+            HIP_TRY(hipStreamCreateWithFlags(&e->streams[have], hipStreamNonBlocking));
+            bool ok = false;
+            HIP_TRY(streams_concurrent(e->streams, have + 1, e->d_probe.p, (long long)(5 * (long long)khz), &ok));
+            if (ok) {
+                ++have;
+            } else {
+                aside.push_back(e->streams[have]);
+                e->streams[have] = nullptr;
+            }
+        }
+        for (auto st : aside) (void)hipStreamDestroy(st);
+        e->lanes_distinct = have;
+    }
     for (auto &st : e->streams)
         if (!st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     return 0;
@@ -994,8 +1020,8 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             int np = 0;
             for (int u = 0; u < e->n_units; ++u) np += promoted[u] && ctl[u].status == ST_RUNNING;
             fprintf(stderr, "[abie sched] round %d t %.3f s: %d units running, %d promoted, %d launched in "
-                    "the arrays, %d in the tail set (%lld bodies), tail %d\n", round, el, n_run_units, np,
-                    (int)ord.size(), e->ts.n, e->ts.total, (int)e->tail);
+                    "the arrays, %d in the tail set (%lld bodies), tail %d, %d distinct lanes\n", round, el,
+                    n_run_units, np, (int)ord.size(), e->ts.n, e->ts.total, (int)e->tail, e->lanes_distinct);
             if (const char *ts = getenv("ABIE_HIP_TRACE_SYS")) {
                 const int sy = atoi(ts);
                 if (sy >= 0 && sy < e->n_sys) {
@@ -1127,7 +1153,7 @@ void abie_ensemble_destroy(abie_ensemble *e) {
     e->d_sys.release(); e->d_order.release(); e->d_orig.release(); e->d_inv.release();
     e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release(); e->d_mratio.release();
     e->d_link_count.release(); e->d_link_part.release(); e->d_sys_inv.release();
-    e->d_running.release(); e->d_clock.release();
+    e->d_running.release(); e->d_clock.release(); e->d_probe.release();
     e->ts.release();
     for (auto &s : e->streams)
         if (s) (void)hipStreamDestroy(s);
