@@ -198,10 +198,11 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     // ensemble's makespan: 4,096 planetary systems of 3..64 massive bodies
     // and 0..2,048 test particles ran 0.2 yr at 512, 811, 1,055 system-yr/s
     // with 1024-, 512- and 256-body slices (128: 1,020).
-    // A lone system has the GPU to itself: 128-body slices put more CUs on
-    // it (1 star + 4 planets + 100,000 test particles: 133 against 122-125
-    // system-yr/s with 256).
-    int cap = S == 1 ? 128 : 256;
+    // A lone system too: 1 star + 4 planets + 100,000 test particles, 100 yr,
+    // 2.44 us per evaluation with 256-body slices (399 units), 2.53 with 201
+    // (511), 2.57 with 400 or 512, 2.79 with 160 and 3.46 with 128 (814
+    // units of two waves, placed unevenly over the CUs).
+    int cap = 256;
     if (const char *sl = getenv("ABIE_HIP_SLICE")) cap = std::max(128, atoi(sl));
     // plain controller: systems of more than link_above bodies run as linked
     // slices even when one workgroup would hold them (default: above 256
@@ -209,7 +210,8 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     int link_above = e->link_above;
     if (S == 1 && link_above == (1 << 30)) link_above = 256;  // a lone system: linked slices over more CUs
     if (const char *la = getenv("ABIE_HIP_LINK_ABOVE")) link_above = atoi(la);
-    const int link_cap = e->link_cap > 0 ? e->link_cap : cap;
+    // linked slices of a lone system: 128 bodies (below)
+    const int link_cap = e->link_cap > 0 ? e->link_cap : getenv("ABIE_HIP_SLICE") ? cap : S == 1 ? 128 : cap;
     const bool auto_link_cap = getenv("ABIE_HIP_SLICE") == nullptr;
     const int n_cu = device_cu_count(e->dev);
     link_above = std::max(link_above, link_cap);
@@ -279,13 +281,28 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     auto quad_of = [&](int n) {
         return (e->ctrl != ABIE_CTRL_PLAIN && e->quad != 0 && n <= kQuadMaxBodies) ? pick_quad(n) : -1;
     };
-    // the hybrid schedule runs every one-lane unit of the massive-only
-    // controllers (at most 256 bodies) in the (256, 1) shape: one launch,
-    // so that it never waits on a hardware queue behind another shape's
+    // the hybrid schedule runs the one-lane units of the massive-only
+    // controllers (at most 256 bodies) in one launch, so that none waits on
+    // a hardware queue behind another shape's: in the shape the tuning
+    // rules give them when that is the same for all (a lone system's
+    // slices: 1 star + 4 planets + 100,000 test particles in 128-body
+    // slices fit the GPU at once as (128, 1) groups, not as (256, 1)),
+    // else in the (256, 1) shape
+    auto hybrid_small = [&](int n, int nm) {
+        return e->quad == 1 && quad_of(n) >= 0 && cfg_fits(one_lane_256(), n, nm);
+    };
+    int small_cfg = -1;
+    for (const Unit &un : units) {
+        const int n = un.nm + (un.t1 - un.t0);
+        if (un.cfg >= 0 || !hybrid_small(n, un.nm)) continue;
+        int c = pick_cfg(n, un.nm);
+        if (c < 0) c = one_lane_256();
+        small_cfg = (small_cfg == -1 || small_cfg == c) ? c : one_lane_256();
+    }
     auto unit_cfg = [&](int n, int nm) {
         const int q = quad_of(n);
         if (q >= 0 && e->quad == 2) return kNumCfg + q;
-        if (q >= 0 && e->quad == 1 && cfg_fits(one_lane_256(), n, nm)) return one_lane_256();
+        if (hybrid_small(n, nm)) return small_cfg;
         return pick_cfg(n, nm);
     };
     const int U = (int)units.size();
