@@ -876,19 +876,32 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     // growing length (0.25 s, then as long as the run so far, at most 1 s);
     // after each, a unit whose projected time to t_end (its model time left
     // over its progress rate while it ran in the round) exceeds 4 times the
-    // median is promoted for the rest of the run, most remaining time first:
-    // it runs in the quad form, in a tail set of at most a quarter of the CUs
-    // (one per unit, the rest splitting the units with the most remaining
-    // time by the rows rule), launched ahead of the one-lane units.  Its
-    // trajectory does not change (the same bits in either form).
+    // median is promoted, most remaining time first, while it stays slower
+    // than the median: it runs in the quad form, in a tail set of at most a
+    // quarter of the CUs (one per unit, the rest splitting the units with
+    // the most remaining time by the rows rule), launched ahead of the
+    // one-lane units.  Its trajectory does not change (the same bits in
+    // either form).
     const bool promoting = splitting && e->promote;
     int promote_cus = std::max(1, n_cu / 4);
     if (const char *s = getenv("ABIE_HIP_PROMOTE_CUS")) promote_cus = std::max(1, std::min(n_cu, atoi(s)));
-    int promote_units = promote_cus;
+    int promote_units = std::max(1, promote_cus / 2);
     if (const char *s = getenv("ABIE_HIP_PROMOTE_UNITS")) promote_units = std::max(1, std::min(promote_cus, atoi(s)));
     double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
     if (const char *s = getenv("ABIE_HIP_PROMOTE_FACTOR")) promote_factor = atof(s);
     if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
+    // Start order (more units than the GPU holds at once): the units wait
+    // in a queue, so a unit that needs many more steps per model year than
+    // its size suggests, and starts late, ends the run alone.  A first round
+    // of a few attempts per unit measures each one's model time per attempt
+    // and run time per attempt; the queue is then ordered by the projected
+    // run time left, longest first (LPT on measured times), and the
+    // promotion below can judge every unit after that round.  Units stop and
+    // resume between rounds with the same bits.
+    int probe_attempts = n_plain_units > n_cu ? 16 : 0;
+    if (const char *s = getenv("ABIE_HIP_PROBE_ATTEMPTS")) probe_attempts = std::max(0, atoi(s));
+    bool probe_order = true;
+    if (const char *s = getenv("ABIE_HIP_PROBE_ORDER")) probe_order = atoi(s) != 0;
     std::vector<char> promoted(e->n_units, 0);
     std::vector<double> t_prev(e->n_units, 0.0);
     std::vector<long long> busy_prev(e->n_units, 0);
@@ -915,6 +928,20 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         }
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (max_seconds > 0 && el >= max_seconds) break;
+        if (round == 1 && probe_attempts > 0 && probe_order) {
+            std::vector<double> key(e->n_units, 0.0);
+            double worst = 0.0;
+            for (int u = 0; u < e->n_units; ++u) {
+                const double att = (double)(ctl[u].n_attempts - att_run0[u]), dt = ctl[u].t - t_run0[u];
+                if (ctl[u].status != ST_RUNNING || att <= 0) continue;
+                const double busy = (double)(ctl[u].busy_ticks - busy_prev[u]);  // ticks in the first round
+                key[u] = dt > 0 ? (ctl[u].t_end - ctl[u].t) / (dt / att) * (busy / att) : -1.0;
+                worst = std::max(worst, key[u]);
+            }
+            for (auto &k : key)
+                if (k < 0) k = 2.0 * worst + 1.0;  // no model time yet: first
+            std::stable_sort(e->lpt.begin(), e->lpt.end(), [&](int a, int b) { return key[a] > key[b]; });
+        }
         split.assign(e->n_units, 0);
         if (promoting && !e->tail && round > 0 && el > el_prev) {
             // only units that ran in the last round have a rate (with more
@@ -924,7 +951,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             // projected run time left: model time left over the rate per
             // tick the unit was running in the last round (units that ran for
             // less than a tenth of the round are not judged)
-            const double min_ticks = 0.1 * w * 1e3 * (double)e->clock_khz;
+            const double min_ticks = (round == 1 && probe_attempts > 0) ? 1.0 : 0.1 * w * 1e3 * (double)e->clock_khz;
             for (int u = 0; u < e->n_units; ++u) {
                 if (ctl[u].status != ST_RUNNING || e->desc[u].link_n > 1) continue;
                 const double busy = (double)(ctl[u].busy_ticks - busy_prev[u]);
@@ -936,12 +963,21 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             if (!all.empty()) {
                 std::nth_element(all.begin(), all.begin() + all.size() / 2, all.end());
                 const double med = all[all.size() / 2];
+                // the set is ranked afresh each round: a unit above the
+                // factor joins, a promoted one (measured in the faster form)
+                // stays while it is slower than the median and ranks at twice
+                // its projection; the rest go back to the one-lane form
                 int live = 0;
                 std::vector<std::pair<double, int>> cand;
                 for (int u = 0; u < e->n_units; ++u) {
-                    if (pu[u] < 0) continue;
-                    if (promoted[u]) ++live;
-                    else if (e->alt_quad[u] >= 0 && pu[u] > promote_factor * med) cand.push_back({pu[u], u});
+                    if (pu[u] < 0) {
+                        live += promoted[u] && ctl[u].status == ST_RUNNING;  // not judged: kept
+                        continue;
+                    }
+                    const bool was = promoted[u] != 0;
+                    promoted[u] = 0;
+                    if (e->alt_quad[u] < 0) continue;
+                    if (was ? pu[u] > med : pu[u] > promote_factor * med) cand.push_back({pu[u] * (was ? 2.0 : 1.0), u});
                 }
                 std::sort(cand.rbegin(), cand.rend());
                 // one CU per promoted unit; CUs left over split the ones
@@ -1014,6 +1050,7 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
         const int n_launch = n_run - (n_run_units - (int)ord.size()) + e->ts.n;  // split parents run as children
         HIP_TRY(hipMemcpy(e->d_running.p, &n_launch, sizeof(int), hipMemcpyHostToDevice));
         GR15Args a = ens_args(e);
+        if (round == 0 && probe_attempts > 0) a.max_attempts = std::min(a.max_attempts, probe_attempts);
         a.budget_ticks = (long long)(group_seconds * 1e3 * (double)e->clock_khz);
         a.running = e->d_running.p;
         // one-lane units yield to the quad form at the tail; at the tail a
