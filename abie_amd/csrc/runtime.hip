@@ -898,7 +898,16 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     // run time left, longest first (LPT on measured times), and the
     // promotion below can judge every unit after that round.  Units stop and
     // resume between rounds with the same bits.
-    int probe_attempts = n_plain_units > n_cu ? 16 : 0;
+    // (only when the units of some shape outnumber the groups of that shape
+    // the GPU holds at once: otherwise every unit starts at once)
+    int probe_attempts = 0;
+    {
+        std::vector<long long> per_cfg(kNumAllCfg, 0);
+        for (int u = 0; u < e->n_units; ++u)
+            if (e->desc[u].link_n <= 1) ++per_cfg[e->cur_cfg[u]];
+        for (int c = 0; c < kNumCfg; ++c)
+            if (per_cfg[c] > (long long)n_cu * gr15_blocks_per_cu(c)) probe_attempts = 16;
+    }
     if (const char *s = getenv("ABIE_HIP_PROBE_ATTEMPTS")) probe_attempts = std::max(0, atoi(s));
     bool probe_order = true;
     if (const char *s = getenv("ABIE_HIP_PROBE_ORDER")) probe_order = atoi(s) != 0;
