@@ -472,11 +472,13 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         c.status = ST_RUNNING;
     }
     HIP_TRY(hipMemcpy(e->d_ctl.p, ctl.data(), U * sizeof(SysCtl), hipMemcpyHostToDevice));
-    if (!e->streams[0]) {
+    if (!e->streams[0] && e->group_off[kNumAllCfg] > 0) {
         // four lanes on distinct hardware queues: each new stream joins the
         // lanes if all of them then run a probe kernel at the same time;
         // one that shares a queue with a lane is kept aside (holding its
-        // queue, so that the next stream goes elsewhere) and released after
+        // queue, so that the next stream goes elsewhere) and released after.
+        // (Not for an ensemble of linked slices only, such as the drop-in's
+        // engine: those launch on one lane.)
         int khz = 0;
         HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->dev));
         HIP_TRY(e->d_probe.reserve(2));
@@ -485,7 +487,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         for (int attempt = 0; attempt < 16 && have < abie_ensemble::kLanes; ++attempt) {
             HIP_TRY(hipStreamCreateWithFlags(&e->streams[have], hipStreamNonBlocking));
             bool ok = false;
-            HIP_TRY(streams_concurrent(e->streams, have + 1, e->d_probe.p, (long long)(5 * (long long)khz), &ok));
+            HIP_TRY(streams_concurrent(e->streams, have + 1, e->d_probe.p, (long long)(2 * (long long)khz), &ok));
             if (ok) {
                 ++have;
             } else {
