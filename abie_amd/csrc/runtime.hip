@@ -372,6 +372,42 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         for (int r = 0; r < ln; ++r) e->desc[u + r].sys_inv0 = first;
         u += ln;
     }
+    // Work per unit for the start order (the queue of units waiting for a
+    // group slot, longest first): evaluations per model year taken as
+    // proportional to 1 / tau, tau = min over massive pairs of
+    // sqrt(r^3 / G (m_i + m_j)) at the initial positions (the shortest
+    // orbital time scale, which sets the step), times the work of an
+    // evaluation, n (N_m + 8) (8 sources' worth of per-body step work).  A
+    // system with close pairs needs 30-100 times the median evaluations per
+    // model year; ordered by size alone it starts late and ends the run
+    // alone.  (4,096 planetary systems, learned proposer, mixed forces:
+    // 0.2 / 1 / 10 yr in 0.222 / 0.738 / 6.84 s against 0.235 / 0.955 /
+    // 8.17 s by size.)  ABIE_HIP_ORDER=0: by size, n N_m.
+    std::vector<double> uwork(U);
+    {
+        int order_mode = 1;
+        if (const char *s = getenv("ABIE_HIP_ORDER")) order_mode = atoi(s);
+        std::vector<double> tau(S, 1.0);
+        if (order_mode == 1)
+            for (int sy = 0; sy < S; ++sy) {
+                const int o = offsets[sy], n = offsets[sy + 1] - o;
+                double best = DBL_MAX;
+                for (int i = 0; i < n; ++i) {
+                    if (!(m[o + i] > 0.0)) continue;
+                    for (int j = i + 1; j < n; ++j) {
+                        if (!(m[o + j] > 0.0)) continue;
+                        const double dx = pos[3 * (o + i)] - pos[3 * (o + j)], dy = pos[3 * (o + i) + 1] - pos[3 * (o + j) + 1],
+                                     dz = pos[3 * (o + i) + 2] - pos[3 * (o + j) + 2];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        best = std::min(best, r2 * std::sqrt(r2) / (e->G * (m[o + i] + m[o + j])));
+                    }
+                }
+                tau[sy] = best < DBL_MAX && best > 0.0 ? std::sqrt(best) : 1.0;
+            }
+        for (int u = 0; u < U; ++u)
+            uwork[u] = order_mode == 1 ? (double)e->desc[u].n * (e->desc[u].nm + 8) / tau[e->unit_sys[u]]
+                                       : (double)e->desc[u].n * e->desc[u].nm;
+    }
     // LPT order inside each configuration group (largest work first)
     e->order.clear();
     for (int c = 0; c < kNumAllCfg; ++c) {
@@ -379,17 +415,12 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         std::vector<int> grp;
         for (int u = 0; u < U; ++u)
             if (e->desc[u].cfg == c && e->desc[u].link_n <= 1) grp.push_back(u);
-        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) {
-            const double wa = (double)e->desc[a].n * e->desc[a].nm, wb = (double)e->desc[b].n * e->desc[b].nm;
-            return wa > wb;
-        });
+        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) { return uwork[a] > uwork[b]; });
         e->order.insert(e->order.end(), grp.begin(), grp.end());
     }
     e->group_off[kNumAllCfg] = (int)e->order.size();
     e->lpt.assign(e->order.begin(), e->order.end());
-    std::stable_sort(e->lpt.begin(), e->lpt.end(), [&](int a, int b) {
-        return (double)e->desc[a].n * e->desc[a].nm > (double)e->desc[b].n * e->desc[b].nm;
-    });
+    std::stable_sort(e->lpt.begin(), e->lpt.end(), [&](int a, int b) { return uwork[a] > uwork[b]; });
     e->cur_cfg.resize(U);
     e->alt_quad.resize(U);
     for (int u = 0; u < U; ++u) {
@@ -895,29 +926,26 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
     if (const char *s = getenv("ABIE_HIP_PROMOTE_FACTOR")) promote_factor = atof(s);
     if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
-    // Start order (more units than the GPU holds at once): the units wait
-    // in a queue, so a unit that needs many more steps per model year than
-    // its size suggests, and starts late, ends the run alone.  A first round
-    // of a few attempts per unit measures each one's model time per attempt
-    // and run time per attempt; the queue is then ordered by the projected
-    // run time left, longest first (LPT on measured times), and the
-    // promotion below can judge every unit after that round.  Units stop and
-    // resume between rounds with the same bits.
-    // (under the hybrid schedule, and only when the units of some shape
-    // outnumber the groups of that shape the GPU holds at once: otherwise
-    // every unit starts at once.  The plain controller's whole-system units
-    // are not probed: its 4,096-system ensemble, ordered by size already,
-    // ran 0.2 yr in 2.23 s with the probe round against 2.08 s without, the
-    // largest system waiting out the round after its 16 attempts.)
+    // Probe round (ABIE_HIP_PROBE_ATTEMPTS=n, hybrid schedule with
+    // promotion, when the units of some shape outnumber the groups of that
+    // shape the GPU holds at once): a first round of n attempts per unit
+    // measures each one's model time and run time per attempt; the queue is
+    // then re-sorted by the projected run time left (LPT on measured
+    // times), and the promotion below can judge every unit after that
+    // round.  Units stop and resume between rounds with the same bits.  The
+    // start order from the initial orbital time scales (ens_build) does as
+    // well without the round's cost: 4,096 planetary systems, learned
+    // proposer, mixed forces, 0.2 / 1 / 10 / 100 yr in 0.26 / 0.86 / 6.98 /
+    // 66.7 s with 16 probe attempts against 0.222 / 0.738 / 6.84 / 67.7 s.
     int probe_attempts = 0;
-    if (promoting) {
+    if (promoting && getenv("ABIE_HIP_PROBE_ATTEMPTS")) {
         std::vector<long long> per_cfg(kNumAllCfg, 0);
         for (int u = 0; u < e->n_units; ++u)
             if (e->desc[u].link_n <= 1) ++per_cfg[e->cur_cfg[u]];
         for (int c = 0; c < kNumCfg; ++c)
-            if (per_cfg[c] > (long long)n_cu * gr15_blocks_per_cu(c)) probe_attempts = 16;
+            if (per_cfg[c] > (long long)n_cu * gr15_blocks_per_cu(c))
+                probe_attempts = std::max(0, atoi(getenv("ABIE_HIP_PROBE_ATTEMPTS")));
     }
-    if (const char *s = getenv("ABIE_HIP_PROBE_ATTEMPTS")) probe_attempts = std::max(0, atoi(s));
     bool probe_order = true;
     if (const char *s = getenv("ABIE_HIP_PROBE_ORDER")) probe_order = atoi(s) != 0;
     std::vector<char> promoted(e->n_units, 0);
