@@ -168,6 +168,7 @@ struct abie_ensemble {
     static constexpr int kLanes = 4;
     hipStream_t streams[kLanes] = {};
     int lanes_distinct = 0;            // lanes that passed the concurrency probe
+    std::vector<double> tau_sys;       // shortest orbital time scale per system at the start
     DBuf<unsigned> d_probe;
     DBuf<SysDesc> d_sys;
     DBuf<int> d_order, d_orig, d_inv;
@@ -387,9 +388,9 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
     {
         int order_mode = 1;
         if (const char *s = getenv("ABIE_HIP_ORDER")) order_mode = atoi(s);
-        std::vector<double> tau(S, 1.0);
-        if (order_mode == 1)
-            for (int sy = 0; sy < S; ++sy) {
+        std::vector<double> &tau = e->tau_sys;
+        tau.assign(S, 1.0);
+        for (int sy = 0; sy < S; ++sy) {
                 const int o = offsets[sy], n = offsets[sy + 1] - o;
                 double best = DBL_MAX;
                 for (int i = 0; i < n; ++i) {
@@ -403,7 +404,7 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
                     }
                 }
                 tau[sy] = best < DBL_MAX && best > 0.0 ? std::sqrt(best) : 1.0;
-            }
+        }
         for (int u = 0; u < U; ++u)
             uwork[u] = order_mode == 1 ? (double)e->desc[u].n * (e->desc[u].nm + 8) / tau[e->unit_sys[u]]
                                        : (double)e->desc[u].n * e->desc[u].nm;
@@ -926,25 +927,27 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     double promote_factor = 4.0, round0 = 0.25;  // tests force promotions with short rounds
     if (const char *s = getenv("ABIE_HIP_PROMOTE_FACTOR")) promote_factor = atof(s);
     if (const char *s = getenv("ABIE_HIP_ROUND_SECONDS")) round0 = std::max(1e-3, atof(s));
-    // Probe round (ABIE_HIP_PROBE_ATTEMPTS=n, hybrid schedule with
-    // promotion, when the units of some shape outnumber the groups of that
-    // shape the GPU holds at once): a first round of n attempts per unit
+    // Probe round (hybrid schedule with promotion, when the units of some
+    // shape outnumber the groups of that shape the GPU holds at once; the
+    // attempt count from ABIE_HIP_PROBE_ATTEMPTS, else chosen at the run's
+    // start below): a first round of n attempts per unit
     // measures each one's model time and run time per attempt; the queue is
     // then re-sorted by the projected run time left (LPT on measured
     // times), and the promotion below can judge every unit after that
     // round.  Units stop and resume between rounds with the same bits.  The
-    // start order from the initial orbital time scales (ens_build) does as
-    // well without the round's cost: 4,096 planetary systems, learned
-    // proposer, mixed forces, 0.2 / 1 / 10 / 100 yr in 0.26 / 0.86 / 6.98 /
-    // 66.7 s with 16 probe attempts against 0.222 / 0.738 / 6.84 / 67.7 s.
-    int probe_attempts = 0;
-    if (promoting && getenv("ABIE_HIP_PROBE_ATTEMPTS")) {
+    // start order from the initial orbital time scales (ens_build) costs
+    // nothing but misses a system that turns heavy later: 4,096 planetary
+    // systems, learned proposer, mixed forces, 0.2 / 1 / 10 / 100 / 1,000
+    // yr in 0.26 / 0.86 / 6.98 / 66.7 / 829 s with 16 probe attempts
+    // against 0.222 / 0.738 / 6.84 / 67.7 / 945 s without.
+    int probe_attempts = 0, probe_env = -1;
+    bool probe_fits = false;  // some shape's units outnumber its resident groups
+    if (const char *s = getenv("ABIE_HIP_PROBE_ATTEMPTS")) probe_env = std::max(0, atoi(s));
+    if (promoting) {
         std::vector<long long> per_cfg(kNumAllCfg, 0);
         for (int u = 0; u < e->n_units; ++u)
             if (e->desc[u].link_n <= 1) ++per_cfg[e->cur_cfg[u]];
-        for (int c = 0; c < kNumCfg; ++c)
-            if (per_cfg[c] > (long long)n_cu * gr15_blocks_per_cu(c))
-                probe_attempts = std::max(0, atoi(getenv("ABIE_HIP_PROBE_ATTEMPTS")));
+        for (int c = 0; c < kNumCfg; ++c) probe_fits = probe_fits || per_cfg[c] > (long long)n_cu * gr15_blocks_per_cu(c);
     }
     bool probe_order = true;
     if (const char *s = getenv("ABIE_HIP_PROBE_ORDER")) probe_order = atoi(s) != 0;
@@ -960,6 +963,26 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
             for (int u = 0; u < e->n_units; ++u) {
                 t_run0[u] = ctl[u].t;
                 att_run0[u] = ctl[u].n_attempts;
+            }
+            // by default a long run probes (median span of at least 1,000
+            // shortest orbital time scales: 4,096 planetary systems ran
+            // 1,000 yr in 829 s with the probe round, 945 s without, where
+            // one system that turns heavy after ~24 yr, and ranks mid-queue
+            // by its time scale, started late), a short one does not
+            // (10 yr, median 154 time scales: 6.84 s without, 6.98 s with)
+            if (promoting && probe_fits) {
+                if (probe_env >= 0) {
+                    probe_attempts = probe_env;
+                } else {
+                    std::vector<double> orbits;
+                    for (int u = 0; u < e->n_units; ++u)
+                        if (ctl[u].status == ST_RUNNING && e->desc[u].link_n <= 1)
+                            orbits.push_back((ctl[u].t_end - ctl[u].t) / e->tau_sys[e->unit_sys[u]]);
+                    if (!orbits.empty()) {
+                        std::nth_element(orbits.begin(), orbits.begin() + orbits.size() / 2, orbits.end());
+                        probe_attempts = orbits[orbits.size() / 2] >= 1000.0 ? 16 : 0;
+                    }
+                }
             }
         }
         int n_run = 0, n_run_units = 0;

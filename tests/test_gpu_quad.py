@@ -189,3 +189,32 @@ def test_promoted_stragglers_same_bits(monkeypatch, force_mode, controller):
     assert np.array_equal(r["t"], ref["t"]) and np.array_equal(r["st"], ref["st"])
     assert same_bits(r["p"], ref["p"]) and same_bits(r["v"], ref["v"])
     assert np.array_equal(r["cnt"], ref["cnt"])
+
+
+@pytest.mark.parametrize("force_mode,controller", [("fp64", "massive"), ("mixed", "learned")])
+def test_start_order_and_probe_round_same_bits(monkeypatch, force_mode, controller):
+    """The start order of the queued units (by size, or by predicted work
+    from the initial orbital time scales) and the optional probe round (a
+    first round of a few attempts per unit, then the queue re-sorted by
+    measured rates) change when each unit runs, never what it computes:
+    1,200 small systems (more one-lane units than the GPU holds at once, so
+    the probe round runs), identical bits, times, statuses and counters."""
+    import os
+    from abie_amd import ics
+    w = None
+    if controller == "learned":
+        w = np.load(os.path.join(os.path.dirname(__file__), "..", "abie_amd", "data", "proposer.npz"))["weights"]
+    rng = np.random.default_rng(11)
+    systems = [ics.planetary_system(7100 + k, int(rng.integers(3, 12)), int(rng.integers(0, 120))) for k in range(1200)]
+    monkeypatch.setenv("ABIE_HIP_ORDER", "0")
+    ref = _run(systems, 0.1, force_mode, controller, proposer=w)
+    monkeypatch.setenv("ABIE_HIP_ORDER", "1")
+    r1 = _run(systems, 0.1, force_mode, controller, proposer=w)
+    monkeypatch.setenv("ABIE_HIP_PROBE_ATTEMPTS", "4")
+    r2 = _run(systems, 0.1, force_mode, controller, proposer=w)
+    assert ref["done"]
+    for r in (r1, r2):
+        assert r["done"]
+        assert np.array_equal(r["t"], ref["t"]) and np.array_equal(r["st"], ref["st"])
+        assert same_bits(r["p"], ref["p"]) and same_bits(r["v"], ref["v"])
+        assert np.array_equal(r["cnt"], ref["cnt"])
