@@ -63,6 +63,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
     double *const src = lds;                // [2][NB][4] source lists (x, y, z, G m)
     double *const red = lds + 2 * NB * 4;   // scratch, as gr15_kernel's (kLdsFeat)
     double *const flag = red + 40;
+    int red_par = 0;  // wg_max2's half of the reduction scratch
     const long long t_enter = wall_clock64();
     const int sys = A.order[blockIdx.x];
     const SysDesc d = A.sys[sys];
@@ -157,12 +158,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
         a0 = forces(y0);
         n_evals += 1;
         double d0 = in_norm ? fabs(y0) : -DBL_MAX, d1 = in_norm ? fabs(a0) : -DBL_MAX;
-        wg_max2<NT>(d0, d1, red, nw);
+        wg_max2<NT>(d0, d1, red, red_par, nw);
         const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
         const double acc = forces(y0 + dt0 * dy0);
         n_evals += 1;
         double d2 = in_norm ? fabs(acc - a0) : -DBL_MAX, dummy = -DBL_MAX;
-        wg_max2<NT>(d2, dummy, red, nw);
+        wg_max2<NT>(d2, dummy, red, red_par, nw);
         d2 = d2 / dt0;
         double dt1;
         if (fmax(d1, d2) <= 1e-15) dt1 = fmax(1e-6, dt0 * 1e-3);
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
 #undef ABIE_QNODE
             n_evals += 7;
             double mdb = in_norm ? fabs(bb[6] - b6p) : -DBL_MAX, mf = in_norm ? fabs(acc) : -DBL_MAX;
-            wg_max2<NT>(mdb, mf, red, nw);
+            wg_max2<NT>(mdb, mf, red, red_par, nw);
             const double pc_err = mdb / mf;
             if (pc_err < kTolpc) break;
             // stop once the correction no longer shrinks (Rein & Spiegel 2015)
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
         acc = forces(yp);
         n_evals += 1;
         double mb = in_norm ? fabs(bb[6]) : -DBL_MAX, ma = in_norm ? fabs(acc) : -DBL_MAX;
-        wg_max2<NT>(mb, ma, red, nw);
+        wg_max2<NT>(mb, ma, red, red_par, nw);
         const double estim = (ma > 0.0) ? mb / ma : 0.0;
         const double err = step_err<CT>(estim);
         const double dtreq = h / err;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(quad_wpe(NT)
                     best = fmax(best, pair_rate2(dx, dyy, dz, vx, vy, vz));
                 }
             }
-            wg_max2<NT>(best, unused, red, nw);
+            wg_max2<NT>(best, unused, red, red_par, nw);
             rate2 = best;
         }
 
