@@ -32,8 +32,8 @@ template <int CTRL>
 __device__ __forceinline__ double quad_dpp(double v) {
     const long long u = __double_as_longlong(v);
     const int lo = (int)(unsigned)(u & 0xffffffffll), hi = (int)(u >> 32);
-    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
-    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
     return __longlong_as_double(((long long)rhi << 32) | (long long)(unsigned)rlo);
 }
 // quad_perm encodings: lane k of a quad reads lane sel_k (2 bits each)
