@@ -130,6 +130,34 @@ hipError_t launch_body_gather(const long long *map, long long n, const BodyArray
     return hipGetLastError();
 }
 
+// dst[idx[i]] = src[i] for i < n (control blocks of split units merged back
+// from their children, one launch instead of one copy per unit)
+__global__ void ctl_scatter_kernel(const SysCtl *__restrict__ src, const int *__restrict__ idx, int n,
+                                   SysCtl *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
+
+hipError_t launch_ctl_scatter(const SysCtl *src, const int *idx, int n, SysCtl *dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctl_scatter_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, src, idx, n, dst);
+    return hipGetLastError();
+}
+
+// dst[i] = src[idx[i]] for i < n (the control blocks of the units still
+// running, read back without the rest)
+__global__ void ctl_gather_kernel(const SysCtl *__restrict__ src, const int *__restrict__ idx, int n,
+                                  SysCtl *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+hipError_t launch_ctl_gather(const SysCtl *src, const int *idx, int n, SysCtl *dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctl_gather_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, src, idx, n, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_body_scatter(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
                                hipStream_t st) {
     if (n <= 0) return hipSuccess;

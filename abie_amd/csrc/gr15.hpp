@@ -170,6 +170,10 @@ hipError_t launch_body_gather(const long long *map, long long n, const BodyArray
                               hipStream_t st);
 // dst body map[i] <- src body i for positions, velocities and the state,
 // where map[i] >= 0 (i < n).
+// dst[i] = src[idx[i]], i < n, on stream st
+hipError_t launch_ctl_gather(const SysCtl *src, const int *idx, int n, SysCtl *dst, hipStream_t st);
+// dst[idx[i]] = src[i], i < n, on stream st
+hipError_t launch_ctl_scatter(const SysCtl *src, const int *idx, int n, SysCtl *dst, hipStream_t st);
 hipError_t launch_body_scatter(const long long *map, long long n, const BodyArrays &src, const BodyArrays &dst,
                                hipStream_t st);
 

@@ -118,10 +118,12 @@ struct TailSet {
     DBuf<SysDesc> sys;
     DBuf<SysCtl> ctl;
     DBuf<long long> d_gmap, d_smap;
+    DBuf<SysCtl> merged;               // parents' merged control blocks (ts_finish)
+    DBuf<int> merged_idx;
     void release() {
         for (auto *b : {&x, &y, &z, &vx, &vy, &vz, &gm, &m, &rad, &state, &mratio}) b->release();
         orig.release(); inv.release(); order.release(); del.release(); sys.release(); ctl.release();
-        d_gmap.release(); d_smap.release();
+        d_gmap.release(); d_smap.release(); merged.release(); merged_idx.release();
     }
 };
 
@@ -168,6 +170,8 @@ struct abie_ensemble {
     static constexpr int kLanes = 4;
     hipStream_t streams[kLanes] = {};
     int lanes_distinct = 0;            // lanes that passed the concurrency probe
+    DBuf<int> d_live;                  // running units' indices (ens_run's tail rounds)
+    DBuf<SysCtl> d_live_ctl;
     std::vector<double> tau_sys;       // shortest orbital time scale per system at the start
     DBuf<unsigned> d_probe;
     DBuf<SysDesc> d_sys;
@@ -853,7 +857,10 @@ static int ts_finish(abie_ensemble *e, const GR15Args &base) {
         for (auto st : e->streams) HIP_TRY(hipStreamSynchronize(st));
     }
     HIP_TRY(launch_body_scatter(t.d_smap.p, t.total, tail_bodies(t), main_bodies(e), e->streams[0]));
-    HIP_TRY(hipStreamSynchronize(e->streams[0]));
+    // each parent's control block from its children (child 0's, with the
+    // earliest start and the latest finish), written back in one scatter
+    std::vector<SysCtl> merged;
+    std::vector<int> idx;
     for (int c = 0; c < t.n;) {
         const int u = t.parent[c];
         SysCtl pc = cc[c];
@@ -864,9 +871,17 @@ static int ts_finish(abie_ensemble *e, const GR15Args &base) {
             pc.done_tick = std::max(pc.done_tick, cc[c1].done_tick);
         }
         pc.stop_attempt = 0;
-        HIP_TRY(hipMemcpy(e->d_ctl.p + u, &pc, sizeof(SysCtl), hipMemcpyHostToDevice));
+        merged.push_back(pc);
+        idx.push_back(u);
         c = c1;
     }
+    const int np = (int)merged.size();
+    HIP_TRY(t.merged.reserve(np));
+    HIP_TRY(t.merged_idx.reserve(np));
+    HIP_TRY(hipMemcpyAsync(t.merged.p, merged.data(), np * sizeof(SysCtl), hipMemcpyHostToDevice, e->streams[0]));
+    HIP_TRY(hipMemcpyAsync(t.merged_idx.p, idx.data(), np * sizeof(int), hipMemcpyHostToDevice, e->streams[0]));
+    HIP_TRY(launch_ctl_scatter(t.merged.p, t.merged_idx.p, np, e->d_ctl.p, e->streams[0]));
+    HIP_TRY(hipStreamSynchronize(e->streams[0]));
     t.n = 0;
     return 0;
 }
@@ -955,8 +970,29 @@ static int ens_run(abie_ensemble *e, double max_seconds, int *done_out) {
     std::vector<double> t_prev(e->n_units, 0.0);
     std::vector<long long> busy_prev(e->n_units, 0);
     double el_prev = 0.0, round_s = round0;
+    // control blocks each round: all of them, or, once few units run (the
+    // tail), only those that ran in the last round (a finished unit's block
+    // no longer changes), gathered on the device and read back in one copy
+    std::vector<int> live;
+    std::vector<SysCtl> live_ctl;
     for (int round = 0;; ++round) {
-        HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        if (round == 0 || live.empty() || (long long)live.size() * 8 > e->n_units) {
+            HIP_TRY(hipMemcpy(ctl.data(), e->d_ctl.p, e->n_units * sizeof(SysCtl), hipMemcpyDeviceToHost));
+        } else {
+            const int nl = (int)live.size();
+            HIP_TRY(e->d_live.reserve(nl));
+            HIP_TRY(e->d_live_ctl.reserve(nl));
+            HIP_TRY(hipMemcpy(e->d_live.p, live.data(), nl * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(launch_ctl_gather(e->d_ctl.p, e->d_live.p, nl, e->d_live_ctl.p, e->streams[0]));
+            live_ctl.resize(nl);
+            HIP_TRY(hipMemcpyAsync(live_ctl.data(), e->d_live_ctl.p, nl * sizeof(SysCtl), hipMemcpyDeviceToHost,
+                                   e->streams[0]));
+            HIP_TRY(hipStreamSynchronize(e->streams[0]));
+            for (int i = 0; i < nl; ++i) ctl[live[i]] = live_ctl[i];
+        }
+        live.clear();
+        for (int u = 0; u < e->n_units; ++u)
+            if (ctl[u].status == ST_RUNNING) live.push_back(u);
         if (round == 0) {
             t_run0.resize(e->n_units);
             att_run0.resize(e->n_units);
@@ -1276,7 +1312,7 @@ void abie_ensemble_destroy(abie_ensemble *e) {
     e->d_sys.release(); e->d_order.release(); e->d_orig.release(); e->d_inv.release();
     e->d_ctl.release(); e->d_del.release(); e->d_log_len.release(); e->d_mlp.release(); e->d_mratio.release();
     e->d_link_count.release(); e->d_link_part.release(); e->d_sys_inv.release();
-    e->d_running.release(); e->d_clock.release(); e->d_probe.release();
+    e->d_running.release(); e->d_clock.release(); e->d_probe.release(); e->d_live.release(); e->d_live_ctl.release();
     e->ts.release();
     for (auto &s : e->streams)
         if (s) (void)hipStreamDestroy(s);
