@@ -394,20 +394,28 @@ static int ens_build(abie_ensemble *e, const int *offsets, const double *pos, co
         if (const char *s = getenv("ABIE_HIP_ORDER")) order_mode = atoi(s);
         std::vector<double> &tau = e->tau_sys;
         tau.assign(S, 1.0);
-        for (int sy = 0; sy < S; ++sy) {
-                const int o = offsets[sy], n = offsets[sy + 1] - o;
-                double best = DBL_MAX;
-                for (int i = 0; i < n; ++i) {
-                    if (!(m[o + i] > 0.0)) continue;
-                    for (int j = i + 1; j < n; ++j) {
-                        if (!(m[o + j] > 0.0)) continue;
-                        const double dx = pos[3 * (o + i)] - pos[3 * (o + j)], dy = pos[3 * (o + i) + 1] - pos[3 * (o + j) + 1],
-                                     dz = pos[3 * (o + i) + 2] - pos[3 * (o + j) + 2];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        best = std::min(best, r2 * std::sqrt(r2) / (e->G * (m[o + i] + m[o + j])));
-                    }
+        // (a lone system's units share one time scale, and a system of more
+        // than 1,024 massive bodies would cost O(N_m^2) host time here: then
+        // the order is by size)
+        if (S == 1) order_mode = 0;
+        for (int sy = 0; sy < S && order_mode == 1; ++sy)
+            if (nms[sy] > 1024) order_mode = 0;
+        for (int sy = 0; sy < S && order_mode == 1; ++sy) {
+            // engine order: the system's massive bodies first
+            const int o = offsets[sy], nm = nms[sy];
+            const std::vector<int> &pm = perms[sy];
+            double best = DBL_MAX;
+            for (int a = 0; a < nm; ++a) {
+                const long long i = o + pm[a];
+                for (int b = a + 1; b < nm; ++b) {
+                    const long long j = o + pm[b];
+                    const double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                                 dz = pos[3 * i + 2] - pos[3 * j + 2];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    best = std::min(best, r2 * std::sqrt(r2) / (e->G * (m[i] + m[j])));
                 }
-                tau[sy] = best < DBL_MAX && best > 0.0 ? std::sqrt(best) : 1.0;
+            }
+            tau[sy] = best < DBL_MAX && best > 0.0 ? std::sqrt(best) : 1.0;
         }
         for (int u = 0; u < U; ++u)
             uwork[u] = order_mode == 1 ? (double)e->desc[u].n * (e->desc[u].nm + 8) / tau[e->unit_sys[u]]
